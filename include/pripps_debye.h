@@ -1,0 +1,225 @@
+/*
+ * pripps_debye.h -- C ABI of the MI355X (gfx950) exact Debye engine.
+ *
+ * The engine replaces the exact pairwise Debye sum behind the reference's
+ * `Pripps::debye_model` (mlund/pripps src/lib.rs:287-303 ->
+ * src/debye.rs:78-136).  Everything above that sum -- YAML form factors,
+ * structure parsing, solvent dummy construction -- stays on the caller's side
+ * of this boundary and arrives here as plain arrays: SoA coordinates, kind
+ * ids, and per-(kind, q) form-factor tables in FP64.
+ *
+ * Conventions
+ *   - Every function returns a pripps_status (0 = success).  On failure a
+ *     message for the calling thread is available from pripps_last_error().
+ *   - Inputs are borrowed for the duration of the call; outputs are
+ *     caller-owned arrays.  Handles are opaque.
+ *   - A built model is immutable: pripps_debye_model_intensity / _profiles /
+ *     _fit may be called concurrently on one handle.  The engine serialises
+ *     its own device submissions.
+ *   - Profiles use the reference's DebyeTerms order (src/debye.rs:176-184):
+ *     0 atoms, 1 excluded, 2 hydration, 3 atom_excluded, 4 atom_hydration,
+ *     5 excluded_hydration; layout [6][nq].
+ *   - No torch or HIP types cross this boundary.
+ */
+#ifndef PRIPPS_DEBYE_H
+#define PRIPPS_DEBYE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  1..9 mirror the reference's Error variants
+ * (src/error.rs:16-74) in declaration order; 10.. are engine-only. */
+typedef enum {
+    PRIPPS_OK = 0,
+    PRIPPS_ERR_FILE = 1,                  /* Error::File */
+    PRIPPS_ERR_IO = 2,                    /* Error::Io */
+    PRIPPS_ERR_YAML = 3,                  /* Error::Yaml */
+    PRIPPS_ERR_UNKNOWN_ATOM = 4,          /* Error::UnknownAtom */
+    PRIPPS_ERR_UNDEFINED_FORMFACTOR = 5,  /* Error::UndefinedFormFactor */
+    PRIPPS_ERR_Q_OUT_OF_RANGE = 6,        /* Error::QOutOfRange */
+    PRIPPS_ERR_VALIDATION = 7,            /* Error::Validation */
+    PRIPPS_ERR_USAGE = 8,                 /* Error::Usage */
+    PRIPPS_ERR_INTERNAL = 9,              /* Error::Internal */
+    PRIPPS_ERR_DEVICE = 10,               /* a HIP runtime call failed */
+    PRIPPS_ERR_OUT_OF_MEMORY = 11,        /* device allocation / admission refused */
+    PRIPPS_ERR_NOT_READY = 12             /* async request still running */
+} pripps_status;
+
+/* Arithmetic of the pair sums. */
+typedef enum {
+    /* FP32 pair arithmetic, sin(q_k r) by a stable angle-addition recurrence on
+     * uniform q grids (direct hardware sin on other grids), FP32 sums over at
+     * most `block` pairs carried into FP64 partials, FP64 across tiles. */
+    PRIPPS_FP32_COMPENSATED = 0,
+    /* The same tiled engine in FP64 throughout. */
+    PRIPPS_FP64 = 1,
+    /* Plain FP64: one libm-style sin per pair and q, no recurrence, no tiling
+     * beyond LDS staging -- the straightforward GPU transcription of the
+     * reference loop, kept as a baseline. */
+    PRIPPS_FP64_PLAIN = 2
+} pripps_precision;
+
+/* One structure.  Coordinates in Angstrom, q in 1/Angstrom. */
+typedef struct {
+    int64_t n_atoms;
+    const double *x, *y, *z;           /* [n_atoms] SoA positions */
+    const int32_t *kind;               /* [n_atoms] ids in [0, n_kinds) */
+    int32_t n_kinds;
+    int32_t nq;
+    const double *q;                   /* [nq] q values, >= 0 */
+    const double *atom_ff;             /* [n_kinds][nq] F_k(q_m) of the atoms */
+    const double *excluded_ff;         /* [n_kinds][nq] excluded-volume dummy form
+                                          factor at unit volume scale, or NULL
+                                          (excluded volume disabled) */
+    const double *water_ff;            /* [nq] shared hydration form factor, or NULL */
+    const double *hydration_weight;    /* [n_atoms] per-atom hydration weight, or
+                                          NULL (hydration disabled) */
+} pripps_debye_input;
+
+typedef struct pripps_engine pripps_engine;
+typedef struct pripps_debye_model pripps_debye_model;
+
+/* Kernel tuning; zero fields mean "choose automatically". */
+typedef struct {
+    int32_t q_per_lane;   /* q points per lane strip: 64 or 128 */
+    int32_t block;        /* pairs summed in FP32 before the FP64 carry */
+    int32_t span;         /* atoms per work item along j */
+} pripps_tuning;
+
+/* Result of a fit (reference FitResult, src/fit.rs:238-247). */
+typedef struct {
+    double volume_scale;
+    double contrast_density;
+    double chi2;
+    double scale;
+} pripps_fit_result;
+
+typedef enum { PRIPPS_FIT_NELDER_MEAD = 0, PRIPPS_FIT_GRID = 1 } pripps_fit_method;
+
+/* Timing / work counters of the last build on a model. */
+typedef struct {
+    double kernel_ms;            /* device time of the pair kernels */
+    double total_ms;             /* host wall time of the build call */
+    double pair_q_terms;         /* N(N-1)/2 * nq */
+    int32_t q_per_lane, block, span;
+    int64_t work_items;
+} pripps_build_stats;
+
+/* Thread-local message of the last failed call on this thread ("" if none). */
+const char *pripps_last_error(void);
+
+/* Library version string. */
+const char *pripps_version(void);
+
+/* Engine on HIP device `device`.  Owns a stream and device scratch. */
+int pripps_engine_create(int device, pripps_engine **out);
+int pripps_engine_destroy(pripps_engine *engine);
+/* Free / total device memory as seen by the engine (bytes). */
+int pripps_engine_memory(pripps_engine *engine, uint64_t *free_bytes, uint64_t *total_bytes);
+
+/*
+ * Build a cached Debye model: the six partial profiles of `input`, kept on
+ * the device.  Replaces debye::build_model (src/debye.rs:78-136) called from
+ * Pripps::debye_model (src/lib.rs:287-303).  `tuning` may be NULL.
+ */
+int pripps_debye_model_build(pripps_engine *engine, const pripps_debye_input *input,
+                             int32_t precision, const pripps_tuning *tuning,
+                             pripps_debye_model **out);
+
+/*
+ * Build many models in one persistent launch (structures of uneven size and
+ * q grid share one device work queue).  `out` receives n handles.  The
+ * reference has no batched entry point; its multi-frame driver is the serial
+ * loop of trajectory_average (src/trajectory.rs:167-251).
+ */
+int pripps_debye_model_build_batch(pripps_engine *engine, int32_t n,
+                                   const pripps_debye_input *inputs, int32_t precision,
+                                   const pripps_tuning *tuning, pripps_debye_model **out);
+
+/* Number of q points (DebyeModel::q_values length, src/debye.rs:171-173). */
+int pripps_debye_model_nq(const pripps_debye_model *model, int32_t *nq);
+/* The q grid the model was built over (src/debye.rs:171-173). */
+int pripps_debye_model_q_values(const pripps_debye_model *model, double *out);
+
+/*
+ * I(q) at (volume_scale, contrast_density): total plus the unscaled self
+ * terms, DebyeModel::intensity (src/debye.rs:146-168) with
+ * DebyeTerms::combine (src/debye.rs:186-195).  Any output may be NULL.
+ */
+int pripps_debye_model_intensity(const pripps_debye_model *model, double volume_scale,
+                                 double contrast_density, double *total, double *atoms,
+                                 double *excluded, double *hydration);
+
+/* The six cached profiles, [6][nq] (the reference caches them in DebyeTerms
+ * but does not expose the cross terms). */
+int pripps_debye_model_profiles(const pripps_debye_model *model, double *out);
+
+/*
+ * Fit (volume_scale, contrast_density) to experimental data on the device
+ * (FP64): reference fit_parameters_with_bounds (src/fit.rs:365-411) and
+ * fit_parameters_grid_with_bounds (src/fit.rs:446-516) over the closure
+ * `model.intensity` (pripps-py/src/lib.rs:625-650, 781-803).  `sigma` may be
+ * NULL (sigma = 1).  Bounds: pointers to (lo, hi) or NULL for the defaults
+ * (0.5, 2.0) and (0.0, 0.1).
+ */
+int pripps_debye_model_fit(const pripps_debye_model *model, const double *q,
+                           const double *intensity, const double *sigma, int64_t n,
+                           int32_t method, int32_t c1_points, int32_t c2_points,
+                           const double *c1_bounds, const double *c2_bounds,
+                           pripps_fit_result *out);
+
+/*
+ * A model from previously exported profiles ([6][nq], pripps_debye_model_profiles)
+ * and their q grid: checkpoint / resume of a cached model without rebuilding.
+ * The reference's DebyeModel is not serialisable (src/debye.rs:139-142); its
+ * fit functions take any (c1, c2) -> Intensity closure (src/fit.rs:365-516),
+ * and a model restored from chosen profiles plays that role here.
+ */
+int pripps_debye_model_from_profiles(pripps_engine *engine, const double *q, int32_t nq,
+                                     const double *profiles, pripps_debye_model **out);
+
+int pripps_debye_model_stats(const pripps_debye_model *model, pripps_build_stats *out);
+
+/*
+ * Re-run the pair and finalize kernels of the engine's most recent build
+ * `repeats` times on its device-resident inputs (no host work, no upload) and
+ * report the mean device time of the pair kernel per run (HIP events on the
+ * engine stream).  `model` must belong to that build (any model of a batch);
+ * otherwise PRIPPS_ERR_USAGE.  Results are bitwise those of the build.  No
+ * reference counterpart: it exists to measure the kernel with inputs in HBM.
+ */
+int pripps_debye_model_recompute(pripps_debye_model *model, int32_t repeats,
+                                 double *pair_ms_mean, double *total_ms);
+int pripps_debye_model_free(pripps_debye_model *model);
+
+/* ---------------------------------------------------------------------- */
+/* Serving: continuous batching with admission by free device memory.      */
+/* ---------------------------------------------------------------------- */
+typedef struct pripps_server pripps_server;
+
+typedef struct {
+    int32_t precision;          /* pripps_precision of every request */
+    int32_t max_in_flight;      /* requests on the device at once (0 = auto) */
+    uint64_t memory_budget;     /* bytes of device memory for requests (0 = 90% of free) */
+} pripps_server_config;
+
+int pripps_server_create(pripps_engine *engine, const pripps_server_config *config,
+                         pripps_server **out);
+/* Queue a build; returns at once with a request id.  The input arrays are
+ * copied before the call returns. */
+int pripps_server_submit(pripps_server *server, const pripps_debye_input *input,
+                         int64_t *request_id);
+/* Wait up to timeout_ms (negative = forever) for a request.  On success the
+ * caller owns *model.  PRIPPS_ERR_NOT_READY when the wait timed out. */
+int pripps_server_wait(pripps_server *server, int64_t request_id, int32_t timeout_ms,
+                       pripps_debye_model **model, double *latency_ms);
+int pripps_server_destroy(pripps_server *server);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PRIPPS_DEBYE_H */

@@ -1,0 +1,640 @@
+// HIP kernels of the exact Debye engine (gfx950).
+//
+// Work decomposition
+//   Atoms are sorted by (kind, hydrated) into class segments.  A work item is
+//   one wave: a 64-atom i-tile of a single kind (one atom per lane) against a
+//   span of j atoms, for one strip of Q q-points held by every lane.  Only
+//   pairs with j > i are taken, so each unordered pair of a structure is
+//   visited once (the upper triangle of the N x N pair matrix).
+//
+// Per pair and strip (uniform q grid, q_k = q_s + k dq):
+//   a = q_s r, h = dq r / 2,  s_0 = sin(a)/r,  delta_1 = 2 cos(a + h) sin(h)/r,
+//   lambda = 4 sin^2(h), then for k = 0..Q-1
+//       acc0[k] += s;  accw[k] += w_j s;  s += delta;  delta -= lambda s
+//   which is the angle-addition recurrence sin(a + (k+1)d) = 2 cos(d)
+//   sin(a + kd) - sin(a + (k-1)d) written in Reinsch's difference form: the
+//   constant lambda keeps its full relative accuracy for small d, where the
+//   plain 2 cos(d) coefficient would lose it, so the FP32 phase does not drift.
+//   Every strip is seeded afresh from hardware sin/cos, so the recurrence is
+//   refreshed every Q steps.  On non-uniform grids each term takes a direct
+//   hardware sin instead.
+//
+// Accumulation
+//   Lanes sum at most `block` pairs in FP32 (acc0/accw), then a cross-lane
+//   reduce-scatter leaves each lane the wave sums of Q/64 q-points, which are
+//   multiplied by the kind form factors and added to six FP64 profile
+//   accumulators.  Each item writes its FP64 partial profiles to its own slot;
+//   a finalize kernel sums the slots in a fixed order (deterministic), divides
+//   by q, and adds the diagonal.  Profiles never need the reference's 4.5 N^2
+//   sinc count: the six profiles come from the colocated-set identity
+//     sum_ij fa_i fb_j s_ij = sum_i fa_i fb_i + sum_{i<j} (fa_i fb_j + fa_j fb_i) s_ij.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "internal.h"
+
+namespace pripps {
+
+constexpr double kInv2Pi = 0.15915494309189533577;  // 1 / (2 pi)
+
+// Read-only data shared by a whole wave (atoms, items, structure records) is
+// read through the constant address space so that uniform indices become
+// scalar loads into SGPRs instead of 64 identical vector loads.
+#define PRIPPS_CONST __attribute__((address_space(4)))
+
+template <typename T>
+__device__ __forceinline__ const PRIPPS_CONST T *as_const(const T *p) {
+    return (const PRIPPS_CONST T *)(p);
+}
+
+template <typename T>
+__device__ __forceinline__ Atom<T> load_atom(const PRIPPS_CONST Atom<T> *p, int j) {
+    Atom<T> a;
+    a.x = p[j].x;
+    a.y = p[j].y;
+    a.z = p[j].z;
+    a.w = p[j].w;
+    return a;
+}
+
+__device__ __forceinline__ Item load_item(const Item *items, int64_t w) {
+    const PRIPPS_CONST Item *p = as_const(items) + w;
+    Item it;
+    it.s = p->s;
+    it.i0 = p->i0;
+    it.icount = p->icount;
+    it.ikind = p->ikind;
+    it.j0 = p->j0;
+    it.j1 = p->j1;
+    it.strip = p->strip;
+    it.pad = 0;
+    return it;
+}
+
+// ---------------------------------------------------------------------------
+// cross-lane helpers
+// ---------------------------------------------------------------------------
+template <int Q>
+struct Split {
+    // values per lane after the reduce-scatter
+    static constexpr int M = Q >= 64 ? Q / 64 : 1;
+    // lanes sharing one q-point when Q < 64
+    static constexpr int kShare = Q >= 64 ? 1 : 64 / Q;
+};
+
+// q-local index held in slot t by `lane` after reduce_scatter.
+template <int Q>
+__device__ __forceinline__ int split_index(int lane, int t) {
+    if constexpr (Q >= 64) return (Q / 64) * lane + t;
+    else return lane / (64 / Q);
+}
+
+template <int Q>
+__device__ __forceinline__ bool split_owner(int lane) {
+    if constexpr (Q >= 64) return true;
+    else return (lane % (64 / Q)) == 0;
+}
+
+// Sum in[0..Q) (optionally times a per-lane scale) over the 64 lanes; lane L
+// receives the totals of the q-local indices split_index<Q>(L, t).
+template <typename T, int Q, bool SCALED>
+__device__ __forceinline__ void reduce_scatter(const T (&in)[Q], T scale, T (&out)[Split<Q>::M]) {
+    const int lane = __lane_id();
+    constexpr int H = Q / 2;
+    T buf[H];
+    {
+        const bool up = lane & 32;
+#pragma unroll
+        for (int t = 0; t < H; ++t) {
+            T lo = in[t], hi = in[t + H];
+            if constexpr (SCALED) {
+                lo *= scale;
+                hi *= scale;
+            }
+            T keep = up ? hi : lo;
+            T send = up ? lo : hi;
+            buf[t] = keep + __shfl_xor(send, 32, 64);
+        }
+    }
+    int len = H;
+#pragma unroll
+    for (int o = 16; o >= 1; o >>= 1) {
+        if (len > 1) {
+            const bool up = lane & o;
+            const int half = len / 2;
+#pragma unroll
+            for (int t = 0; t < half; ++t) {
+                T lo = buf[t], hi = buf[t + half];
+                T keep = up ? hi : lo;
+                T send = up ? lo : hi;
+                buf[t] = keep + __shfl_xor(send, o, 64);
+            }
+            len = half;
+        } else {
+            buf[0] += __shfl_xor(buf[0], o, 64);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < Split<Q>::M; ++t) out[t] = buf[t];
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// per-pair seeds
+// ---------------------------------------------------------------------------
+struct Seed32 {
+    float s, d, lam;
+};
+
+// FP32 seed: q0rev = q_s / 2pi, hrev = dq / 4pi (revolutions per Angstrom).
+__device__ __forceinline__ Seed32 seed_f32(float r2, float q0rev, float hrev, float take) {
+    r2 = fmaxf(r2, 1e-12f);  // coincident atoms: the r -> 0 limit s_k -> q_k
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float r = r2 * rinv;
+    const float arev = q0rev * r;
+    const float hr = hrev * r;
+    const float sa = __builtin_amdgcn_sinf(__builtin_amdgcn_fractf(arev));
+    const float cah = __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(arev + hr));
+    const float sh = __builtin_amdgcn_sinf(hr);
+    const float amp = take * rinv;
+    Seed32 z;
+    z.s = sa * amp;
+    z.d = (2.0f * amp) * (cah * sh);
+    z.lam = 4.0f * (sh * sh);
+    return z;
+}
+
+struct Seed64 {
+    double s, d, lam;
+};
+
+__device__ __forceinline__ Seed64 seed_f64(double r2, double q0, double hdq, double take) {
+    r2 = fmax(r2, 1e-24);
+    const double r = sqrt(r2);
+    const double rinv = 1.0 / r;
+    double sa, ca, sh, ch;
+    sincos(q0 * r, &sa, &ca);
+    sincos(hdq * r, &sh, &ch);
+    const double cah = ca * ch - sa * sh;  // cos(a + h)
+    const double amp = take * rinv;
+    Seed64 z;
+    z.s = sa * amp;
+    z.d = (2.0 * amp) * (cah * sh);
+    z.lam = 4.0 * (sh * sh);
+    return z;
+}
+
+// ---------------------------------------------------------------------------
+// the tiled pair kernel
+// ---------------------------------------------------------------------------
+template <typename T, int Q, bool UNIFORM, bool WET, bool MASK>
+__device__ __forceinline__ void pair_block(const Atom<T> *__restrict__ atoms, int jb, int je,
+                                           const Atom<T> &ai, int iidx, T lane_take,
+                                           const T (&qv)[Q], T q0p, T hp, T (&acc0)[Q],
+                                           T (&accw)[Q]) {
+    const PRIPPS_CONST Atom<T> *ca = as_const(atoms);
+    Atom<T> next = load_atom(ca, jb);
+    for (int j = jb; j < je; ++j) {
+        const Atom<T> aj = next;
+        next = load_atom(ca, min(j + 1, je - 1));  // prefetch the next j atom
+        const T dx = ai.x - aj.x, dy = ai.y - aj.y, dz = ai.z - aj.z;
+        const T r2 = dx * dx + dy * dy + dz * dz;
+        // padding lanes (iidx = INT_MAX, lane_take = 0) never contribute
+        const T take = MASK ? (j > iidx ? T(1) : T(0)) : lane_take;
+        if constexpr (UNIFORM) {
+            T s, d, lam;
+            if constexpr (sizeof(T) == 4) {
+                Seed32 z = seed_f32(r2, q0p, hp, take);
+                s = z.s;
+                d = z.d;
+                lam = z.lam;
+            } else {
+                Seed64 z = seed_f64(r2, q0p, hp, take);
+                s = z.s;
+                d = z.d;
+                lam = z.lam;
+            }
+#pragma unroll
+            for (int k = 0; k < Q; ++k) {
+                acc0[k] += s;
+                if constexpr (WET) accw[k] += aj.w * s;
+                s += d;
+                d -= lam * s;
+            }
+        } else {
+            if constexpr (sizeof(T) == 4) {
+                const float rr2 = fmaxf(r2, 1e-12f);
+                const float rinv = __builtin_amdgcn_rsqf(rr2);
+                const float r = rr2 * rinv;
+                const float amp = take * rinv;
+#pragma unroll
+                for (int k = 0; k < Q; ++k) {
+                    const float s = __builtin_amdgcn_sinf(__builtin_amdgcn_fractf(qv[k] * r)) * amp;
+                    acc0[k] += s;
+                    if constexpr (WET) accw[k] += aj.w * s;
+                }
+            } else {
+                const double rr2 = fmax(r2, 1e-24);
+                const double r = sqrt(rr2);
+                const double amp = take / r;
+#pragma unroll
+                for (int k = 0; k < Q; ++k) {
+                    const double s = sin(qv[k] * r) * amp;
+                    acc0[k] += s;
+                    if constexpr (WET) accw[k] += aj.w * s;
+                }
+            }
+        }
+    }
+}
+
+template <typename T, int Q, bool UNIFORM>
+__global__ void __launch_bounds__(256) debye_tiled_kernel(TiledArgs args) {
+    constexpr int M = Split<Q>::M;
+    const int lane = __lane_id();
+    const bool owner = split_owner<Q>(lane);
+
+    for (;;) {
+        int w = 0;
+        if (lane == 0) w = atomicAdd(args.counter, 1);
+        w = __builtin_amdgcn_readfirstlane(__shfl(w, 0, 64));
+        if (w >= args.n_items) break;
+
+        const Item it = load_item(args.items, w);
+        const PRIPPS_CONST DevStruct &sd = as_const(args.structs)[it.s];
+        const Atom<T> *__restrict__ atoms = static_cast<const Atom<T> *>(sd.atoms);
+        const int nq = sd.nq;
+
+        const bool ivalid = lane < it.icount;
+        const int iidx = ivalid ? it.i0 + lane : 0x7fffffff;
+        Atom<T> ai = atoms[ivalid ? it.i0 + lane : it.i0];
+        const T wi = ivalid ? ai.w : T(0);
+        const T lane_take = ivalid ? T(1) : T(0);
+        const bool wet_i = __ballot(wi != T(0)) != 0;
+
+        const int qbase = it.strip * Q;
+        // strip constants
+        T qv[Q];
+        T q0p = T(0), hp = T(0);
+        if constexpr (UNIFORM) {
+            const double qs = sd.q0 + double(qbase) * sd.dq;
+            if constexpr (sizeof(T) == 4) {
+                q0p = float(qs * kInv2Pi);
+                hp = float(sd.dq * (0.5 * kInv2Pi));
+            } else {
+                q0p = qs;
+                hp = 0.5 * sd.dq;
+            }
+#pragma unroll
+            for (int k = 0; k < Q; ++k) qv[k] = T(0);
+        } else {
+#pragma unroll
+            for (int k = 0; k < Q; ++k) {
+                const int m = min(qbase + k, nq - 1);
+                if constexpr (sizeof(T) == 4) qv[k] = float(sd.q[m] * kInv2Pi);
+                else qv[k] = sd.q[m];
+            }
+        }
+
+        // per-lane form factors of the tile kind at the lane's q-points
+        double ak[M], ek[M], wq[M];
+        double pAA[M], pEE[M], pHH[M], pAE[M], pAH[M], pEH[M];
+#pragma unroll
+        for (int t = 0; t < M; ++t) {
+            const int m = min(qbase + split_index<Q>(lane, t), nq - 1);
+            ak[t] = sd.ffa[it.ikind * nq + m];
+            ek[t] = sd.ffe ? sd.ffe[it.ikind * nq + m] : 0.0;
+            wq[t] = sd.ffw ? sd.ffw[m] : 0.0;
+            pAA[t] = pEE[t] = pHH[t] = pAE[t] = pAH[t] = pEH[t] = 0.0;
+        }
+
+        int j = it.j0;
+        int c = 0;
+        while (j < it.j1) {
+            while (sd.cls_start[c + 1] <= j) ++c;
+            const int jend = min(min(j + args.block, it.j1), sd.cls_start[c + 1]);
+            const int lk = sd.cls_kind[c];
+            const bool wet = sd.cls_wet[c] != 0;
+            const bool mask = j <= it.i0 + it.icount - 1;  // chunk meets the diagonal
+
+            T acc0[Q], accw[Q];
+#pragma unroll
+            for (int k = 0; k < Q; ++k) acc0[k] = accw[k] = T(0);
+
+            if (wet) {
+                if (mask) pair_block<T, Q, UNIFORM, true, true>(atoms, j, jend, ai, iidx, lane_take, qv, q0p, hp, acc0, accw);
+                else pair_block<T, Q, UNIFORM, true, false>(atoms, j, jend, ai, iidx, lane_take, qv, q0p, hp, acc0, accw);
+            } else {
+                if (mask) pair_block<T, Q, UNIFORM, false, true>(atoms, j, jend, ai, iidx, lane_take, qv, q0p, hp, acc0, accw);
+                else pair_block<T, Q, UNIFORM, false, false>(atoms, j, jend, ai, iidx, lane_take, qv, q0p, hp, acc0, accw);
+            }
+
+            // FP64 carry: wave sums of S = sum s, Si = sum w_i s, Sw = sum w_j s,
+            // Siw = sum w_i w_j s for the lane's q-points.
+            T r0[M], ri[M], rw[M], riw[M];
+            reduce_scatter<T, Q, false>(acc0, T(1), r0);
+#pragma unroll
+            for (int t = 0; t < M; ++t) ri[t] = rw[t] = riw[t] = T(0);
+            if (wet_i) reduce_scatter<T, Q, true>(acc0, wi, ri);
+            if (wet) {
+                reduce_scatter<T, Q, false>(accw, T(1), rw);
+                if (wet_i) reduce_scatter<T, Q, true>(accw, wi, riw);
+            }
+#pragma unroll
+            for (int t = 0; t < M; ++t) {
+                const int m = min(qbase + split_index<Q>(lane, t), nq - 1);
+                const double al = sd.ffa[lk * nq + m];
+                const double el = sd.ffe ? sd.ffe[lk * nq + m] : 0.0;
+                const double S = double(r0[t]), Si = double(ri[t]);
+                const double Sw = double(rw[t]), Siw = double(riw[t]);
+                pAA[t] += 2.0 * ak[t] * al * S;
+                pEE[t] += 2.0 * ek[t] * el * S;
+                pAE[t] += (ak[t] * el + al * ek[t]) * S;
+                pHH[t] += 2.0 * wq[t] * wq[t] * Siw;
+                pAH[t] += wq[t] * (ak[t] * Sw + al * Si);
+                pEH[t] += wq[t] * (ek[t] * Sw + el * Si);
+            }
+            j = jend;
+        }
+
+        double *out = args.partials + (int64_t)w * (kProfiles * Q);
+        if (owner) {
+#pragma unroll
+            for (int t = 0; t < M; ++t) {
+                const int ql = split_index<Q>(lane, t);
+                out[0 * Q + ql] = pAA[t];
+                out[1 * Q + ql] = pEE[t];
+                out[2 * Q + ql] = pHH[t];
+                out[3 * Q + ql] = pAE[t];
+                out[4 * Q + ql] = pAH[t];
+                out[5 * Q + ql] = pEH[t];
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// finalize: sum item partials per (structure, strip) in a fixed order, divide
+// by q, add the diagonal; q = 0 takes the closed form (sum f_X)(sum f_Y).
+// ---------------------------------------------------------------------------
+__device__ void diagonal_and_zero(const PRIPPS_CONST DevStruct &sd, int m, double (&diag)[kProfiles],
+                                  double (&zero)[kProfiles]) {
+    double aa = 0, ee = 0, ae = 0, sa = 0, se = 0, aw = 0, ew = 0, sw = 0;
+    for (int k = 0; k < sd.K; ++k) {
+        const double n = sd.kind_n[k];
+        const double a = sd.ffa[k * sd.nq + m];
+        const double e = sd.ffe ? sd.ffe[k * sd.nq + m] : 0.0;
+        const double wk = sd.kind_w[k];
+        aa += n * a * a;
+        ee += n * e * e;
+        ae += n * a * e;
+        sa += n * a;
+        se += n * e;
+        aw += a * wk;
+        ew += e * wk;
+        sw += wk;
+    }
+    const double W = sd.ffw ? sd.ffw[m] : 0.0;
+    diag[0] = aa;
+    diag[1] = ee;
+    diag[2] = W * W * sd.sum_w2;
+    diag[3] = ae;
+    diag[4] = W * aw;
+    diag[5] = W * ew;
+    const double sh = W * sw;
+    zero[0] = sa * sa;
+    zero[1] = se * se;
+    zero[2] = sh * sh;
+    zero[3] = sa * se;
+    zero[4] = sa * sh;
+    zero[5] = se * sh;
+}
+
+__global__ void __launch_bounds__(256) debye_finalize_kernel(FinalizeArgs args) {
+    __shared__ double red[256][kProfiles];
+    const int g_strip = blockIdx.x;
+    const int s = args.strip_struct[g_strip];
+    const int strip = args.strip_index[g_strip];
+    const PRIPPS_CONST DevStruct &sd = as_const(args.structs)[s];
+    const int Q = args.Q;
+    const int G = 256 / Q;
+    const int ql = threadIdx.x % Q;
+    const int g = threadIdx.x / Q;
+    const int b = args.csr_ptr[g_strip], e = args.csr_ptr[g_strip + 1];
+    double acc[kProfiles] = {0, 0, 0, 0, 0, 0};
+    if (g < G) {
+        for (int idx = b + g; idx < e; idx += G) {
+            const double *p = args.partials + (int64_t)args.csr_items[idx] * (kProfiles * Q);
+#pragma unroll
+            for (int k = 0; k < kProfiles; ++k) acc[k] += p[k * Q + ql];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kProfiles; ++k) red[threadIdx.x][k] = acc[k];
+    __syncthreads();
+    if (g == 0) {
+        for (int gg = 1; gg < G; ++gg)
+#pragma unroll
+            for (int k = 0; k < kProfiles; ++k) acc[k] += red[gg * Q + ql][k];
+        const int m = strip * Q + ql;
+        if (m < sd.nq) {
+            double diag[kProfiles], zero[kProfiles];
+            diagonal_and_zero(sd, m, diag, zero);
+            const double q = sd.q[m];
+#pragma unroll
+            for (int k = 0; k < kProfiles; ++k) {
+                const double v = q > 0.0 ? acc[k] / q + diag[k] : zero[k];
+                sd.profiles[k * sd.nq + m] = v;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// plain FP64 path: one sin per pair and q, one wave per (i-tile, q)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) debye_plain_kernel(PlainArgs args) {
+    const PRIPPS_CONST DevStruct &sd = *as_const(args.sd);
+    const int lane = __lane_id();
+    const int m = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int tile = blockIdx.x;
+    if (m >= sd.nq) return;  // whole wave leaves; no block-level barrier below
+    const int4 tl = args.tiles[tile];
+    const Atom<double> *__restrict__ atoms = static_cast<const Atom<double> *>(sd.atoms);
+    const bool ivalid = lane < tl.y;
+    const int iidx = ivalid ? tl.x + lane : 0x7fffffff;
+    const Atom<double> ai = atoms[ivalid ? tl.x + lane : tl.x];
+    const double wi = ivalid ? ai.w : 0.0;
+    const int nq = sd.nq;
+    const double q = sd.q[m];
+    const double ak = sd.ffa[tl.z * nq + m];
+    const double ek = sd.ffe ? sd.ffe[tl.z * nq + m] : 0.0;
+    const double W = sd.ffw ? sd.ffw[m] : 0.0;
+    double pAA = 0, pEE = 0, pHH = 0, pAE = 0, pAH = 0, pEH = 0;
+    int c = 0;
+    for (int j = tl.x; j < sd.n;) {
+        while (sd.cls_start[c + 1] <= j) ++c;
+        const int jend = sd.cls_start[c + 1];
+        const int lk = sd.cls_kind[c];
+        const double al = sd.ffa[lk * nq + m];
+        const double el = sd.ffe ? sd.ffe[lk * nq + m] : 0.0;
+        double T0 = 0.0, Tw = 0.0;
+        for (int jj = j; jj < jend; ++jj) {
+            const Atom<double> aj = load_atom(as_const(atoms), jj);
+            const double dx = ai.x - aj.x, dy = ai.y - aj.y, dz = ai.z - aj.z;
+            const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            const double x = q * r;
+            const double sc = fabs(x) < 1e-8 ? 1.0 : sin(x) / x;
+            const double s = jj > iidx ? sc : 0.0;
+            T0 += s;
+            Tw += aj.w * s;
+        }
+        pAA += 2.0 * ak * al * T0;
+        pEE += 2.0 * ek * el * T0;
+        pAE += (ak * el + al * ek) * T0;
+        pHH += 2.0 * W * W * wi * Tw;
+        pAH += W * (ak * Tw + al * wi * T0);
+        pEH += W * (ek * Tw + el * wi * T0);
+        j = jend;
+    }
+    pAA = wave_sum(pAA);
+    pEE = wave_sum(pEE);
+    pHH = wave_sum(pHH);
+    pAE = wave_sum(pAE);
+    pAH = wave_sum(pAH);
+    pEH = wave_sum(pEH);
+    if (lane == 0) {
+        double *o = args.partials + ((int64_t)tile * nq + m) * kProfiles;
+        o[0] = pAA;
+        o[1] = pEE;
+        o[2] = pHH;
+        o[3] = pAE;
+        o[4] = pAH;
+        o[5] = pEH;
+    }
+}
+
+__global__ void __launch_bounds__(256) debye_plain_finalize_kernel(PlainArgs args) {
+    const PRIPPS_CONST DevStruct &sd = *as_const(args.sd);
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= sd.nq) return;
+    double acc[kProfiles] = {0, 0, 0, 0, 0, 0};
+    for (int t = 0; t < args.n_tiles; ++t) {
+        const double *p = args.partials + ((int64_t)t * sd.nq + m) * kProfiles;
+#pragma unroll
+        for (int k = 0; k < kProfiles; ++k) acc[k] += p[k];
+    }
+    double diag[kProfiles], zero[kProfiles];
+    diagonal_and_zero(sd, m, diag, zero);
+#pragma unroll
+    for (int k = 0; k < kProfiles; ++k) sd.profiles[k * sd.nq + m] = acc[k] + diag[k];
+}
+
+// ---------------------------------------------------------------------------
+// small kernels
+// ---------------------------------------------------------------------------
+__global__ void fill_int_kernel(int *p, int n, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// I(q) = AA + vs^2 EE + cd^2 HH - 2 vs AE + 2 cd AH - 2 vs cd EH, evaluated in
+// the reference's operation order without FMA contraction
+// (src/debye.rs:186-195).  out: [4][nq] total, atoms, excluded, hydration.
+__global__ void combine_kernel(const double *__restrict__ prof, int nq, double vs, double cd,
+                               double *__restrict__ out) {
+#pragma clang fp contract(off)
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nq) return;
+    const double aa = prof[0 * nq + m], ee = prof[1 * nq + m], hh = prof[2 * nq + m];
+    const double ae = prof[3 * nq + m], ah = prof[4 * nq + m], eh = prof[5 * nq + m];
+    const double t1 = aa + vs * vs * ee;
+    const double t2 = t1 + cd * cd * hh;
+    const double t3 = t2 - 2.0 * vs * ae;
+    const double t4 = t3 + 2.0 * cd * ah;
+    const double total = t4 - 2.0 * vs * cd * eh;
+    out[0 * nq + m] = total;
+    out[1 * nq + m] = aa;
+    out[2 * nq + m] = ee;
+    out[3 * nq + m] = hh;
+}
+
+// ---------------------------------------------------------------------------
+// launch wrappers (host side of this translation unit)
+// ---------------------------------------------------------------------------
+template <typename T, int Q, bool U>
+static hipError_t launch_tiled_t(const TiledArgs &a, int blocks, hipStream_t st) {
+    hipLaunchKernelGGL((debye_tiled_kernel<T, Q, U>), dim3(blocks), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_tiled(bool fp64, int Q, bool uniform, const TiledArgs &a, int blocks,
+                        hipStream_t st) {
+    // Instantiated shapes: those whose accumulators fit the 256 VGPRs of a wave.
+    if (!fp64) {
+        if (Q == 32) return uniform ? launch_tiled_t<float, 32, true>(a, blocks, st)
+                                    : launch_tiled_t<float, 32, false>(a, blocks, st);
+        if (Q == 64) return uniform ? launch_tiled_t<float, 64, true>(a, blocks, st)
+                                    : launch_tiled_t<float, 64, false>(a, blocks, st);
+        if (Q == 128 && uniform) return launch_tiled_t<float, 128, true>(a, blocks, st);
+    } else {
+        if (Q == 32) return uniform ? launch_tiled_t<double, 32, true>(a, blocks, st)
+                                    : launch_tiled_t<double, 32, false>(a, blocks, st);
+        if (Q == 64 && uniform) return launch_tiled_t<double, 64, true>(a, blocks, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <typename T, int Q, bool U>
+static hipError_t occupancy_t(int *blocks_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        blocks_per_cu, reinterpret_cast<const void *>(&debye_tiled_kernel<T, Q, U>), 256, 0);
+}
+
+hipError_t tiled_occupancy(bool fp64, int Q, bool uniform, int *blocks_per_cu) {
+    if (!fp64) {
+        if (Q == 32) return uniform ? occupancy_t<float, 32, true>(blocks_per_cu) : occupancy_t<float, 32, false>(blocks_per_cu);
+        if (Q == 64) return uniform ? occupancy_t<float, 64, true>(blocks_per_cu) : occupancy_t<float, 64, false>(blocks_per_cu);
+        if (Q == 128 && uniform) return occupancy_t<float, 128, true>(blocks_per_cu);
+    } else {
+        if (Q == 32) return uniform ? occupancy_t<double, 32, true>(blocks_per_cu) : occupancy_t<double, 32, false>(blocks_per_cu);
+        if (Q == 64 && uniform) return occupancy_t<double, 64, true>(blocks_per_cu);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_finalize(const FinalizeArgs &a, int n_strips_total, hipStream_t st) {
+    hipLaunchKernelGGL(debye_finalize_kernel, dim3(n_strips_total), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_plain(const PlainArgs &a, int nq, hipStream_t st) {
+    if (a.n_tiles > 0) {
+        hipLaunchKernelGGL(debye_plain_kernel, dim3(a.n_tiles, (nq + 3) / 4), dim3(256), 0, st, a);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(debye_plain_finalize_kernel, dim3((nq + 255) / 256), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_int(int *p, int n, int v, hipStream_t st) {
+    hipLaunchKernelGGL(fill_int_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_combine(const double *prof, int nq, double vs, double cd, double *out,
+                          hipStream_t st) {
+    hipLaunchKernelGGL(combine_kernel, dim3((nq + 255) / 256), dim3(256), 0, st, prof, nq, vs, cd,
+                       out);
+    return hipGetLastError();
+}
+
+}  // namespace pripps

@@ -1,0 +1,885 @@
+// Host side of the exact Debye engine: the C ABI declared in
+// include/pripps_debye.h.
+//
+// A build takes the caller's arrays, validates them, sorts atoms by
+// (kind, hydrated) class, centres and converts the coordinates, lays out one
+// work queue of (structure, i-tile, j-span, q-strip) items, uploads
+// everything with a single copy from pinned staging into a grow-only device
+// arena, and runs the pair kernel, the finalize kernel and nothing else.
+// Models keep only their q grid and six profiles on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <atomic>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+#include "pripps_debye.h"
+
+namespace pripps {
+
+// fit (fit_kernels.hip)
+int device_fit(hipStream_t st, const double *d_prof, const double *d_q, int nq, const double *q,
+               const double *inten, const double *sigma, int64_t n, int method, int c1_points,
+               int c2_points, const double *c1b, const double *c2b, pripps_fit_result *out,
+               std::string *err);
+
+}  // namespace pripps
+
+using namespace pripps;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_last_error;
+
+static int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+
+static int hip_fail(hipError_t e, const char *what) {
+    return fail(e == hipErrorOutOfMemory ? PRIPPS_ERR_OUT_OF_MEMORY : PRIPPS_ERR_DEVICE,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(expr)                                    \
+    do {                                                \
+        hipError_t e_ = (expr);                         \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr); \
+    } while (0)
+
+extern "C" const char *pripps_last_error(void) { return g_last_error.c_str(); }
+extern "C" const char *pripps_version(void) { return "pripps_amd 0.1.0 (gfx950)"; }
+
+// ---------------------------------------------------------------------------
+// engine
+// ---------------------------------------------------------------------------
+struct pripps_engine {
+    std::atomic<int> refs{1};  // the creator's handle plus one per live model
+    int device = 0;
+    int n_cu = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::mutex mu;  // serialises every submission on `stream` and the arenas
+    void *arena = nullptr;
+    size_t arena_size = 0;
+    void *staging = nullptr;  // pinned (hipHostMalloc)
+    size_t staging_size = 0;
+    // launch state of the most recent build, valid while the arena holds it
+    uint64_t build_serial = 0;
+    bool last_plain = false, last_fp64 = false, last_uniform = false;
+    int last_Q = 0, last_blocks = 0, last_strips = 0;
+    TiledArgs last_tiled{};
+    FinalizeArgs last_fin{};
+    int *last_counter = nullptr;
+    int64_t last_items = 0;
+};
+
+struct pripps_debye_model {
+    pripps_engine *engine = nullptr;
+    int32_t nq = 0;
+    std::vector<double> q;
+    double *dev = nullptr;  // [6 nq profiles | nq q | 4 nq combine output]
+    pripps_build_stats stats{};
+    uint64_t build_serial = 0;
+    double *d_prof() const { return dev; }
+    double *d_q() const { return dev + 6 * (size_t)nq; }
+    double *d_out() const { return dev + 7 * (size_t)nq; }
+};
+
+static void engine_release(pripps_engine *e);
+
+static int ensure_arena(pripps_engine *e, size_t bytes) {
+    if (bytes <= e->arena_size) return 0;
+    if (e->arena) {
+        hipError_t err = hipFree(e->arena);
+        e->arena = nullptr;
+        e->arena_size = 0;
+        if (err != hipSuccess) return hip_fail(err, "hipFree(arena)");
+    }
+    size_t sz = std::max(bytes, (size_t)64 << 20);
+    hipError_t err = hipMalloc(&e->arena, sz);
+    if (err != hipSuccess) return hip_fail(err, "hipMalloc(arena)");
+    e->arena_size = sz;
+    return 0;
+}
+
+static int ensure_staging(pripps_engine *e, size_t bytes) {
+    if (bytes <= e->staging_size) return 0;
+    if (e->staging) {
+        hipError_t err = hipHostFree(e->staging);
+        e->staging = nullptr;
+        e->staging_size = 0;
+        if (err != hipSuccess) return hip_fail(err, "hipHostFree(staging)");
+    }
+    size_t sz = std::max(bytes, (size_t)16 << 20);
+    hipError_t err = hipHostMalloc(&e->staging, sz, hipHostMallocDefault);
+    if (err != hipSuccess) return hip_fail(err, "hipHostMalloc(staging)");
+    e->staging_size = sz;
+    return 0;
+}
+
+extern "C" int pripps_engine_create(int device, pripps_engine **out) {
+    if (!out) return fail(PRIPPS_ERR_USAGE, "pripps_engine_create: out is NULL");
+    *out = nullptr;
+    int count = 0;
+    HIPCHK(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count)
+        return fail(PRIPPS_ERR_USAGE, "pripps_engine_create: device " + std::to_string(device) +
+                                          " out of range (" + std::to_string(count) + " devices)");
+    HIPCHK(hipSetDevice(device));
+    auto *e = new pripps_engine();
+    e->device = device;
+    hipError_t err = hipDeviceGetAttribute(&e->n_cu, hipDeviceAttributeMultiprocessorCount, device);
+    if (err == hipSuccess) err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (err == hipSuccess) err = hipEventCreate(&e->ev0);
+    if (err == hipSuccess) err = hipEventCreate(&e->ev1);
+    if (err != hipSuccess) {
+        engine_release(e);
+        return hip_fail(err, "pripps_engine_create");
+    }
+    *out = e;
+    return PRIPPS_OK;
+}
+
+static void engine_release(pripps_engine *e) {
+    if (e->refs.fetch_sub(1) != 1) return;
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->arena) (void)hipFree(e->arena);
+    if (e->staging) (void)hipHostFree(e->staging);
+    if (e->ev0) (void)hipEventDestroy(e->ev0);
+    if (e->ev1) (void)hipEventDestroy(e->ev1);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+extern "C" int pripps_engine_destroy(pripps_engine *e) {
+    if (e) engine_release(e);  // models still alive keep the engine until freed
+    return PRIPPS_OK;
+}
+
+extern "C" int pripps_engine_memory(pripps_engine *e, uint64_t *free_b, uint64_t *total_b) {
+    if (!e) return fail(PRIPPS_ERR_USAGE, "pripps_engine_memory: engine is NULL");
+    HIPCHK(hipSetDevice(e->device));
+    size_t f = 0, t = 0;
+    HIPCHK(hipMemGetInfo(&f, &t));
+    if (free_b) *free_b = f;
+    if (total_b) *total_b = t;
+    return PRIPPS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// host preparation of one structure
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Prepared {
+    int32_t n = 0, nq = 0, K = 0;
+    bool has_e = false, has_h = false;
+    bool uniform = false;
+    double q0 = 0, dq = 0;
+    std::vector<double> q;
+    std::vector<double> pos;  // [n][4] centred x y z w, sorted by class
+    std::vector<double> ffa, ffe, ffw, kind_n, kind_w;
+    double sum_w2 = 0;
+    int32_t n_classes = 0;
+    int32_t cls_start[kMaxClasses + 1]{};
+    int8_t cls_kind[kMaxClasses]{};
+    int8_t cls_wet[kMaxClasses]{};
+    std::vector<int4> tiles;  // (i0, count, kind, 0)
+    double pair_q_terms = 0;
+};
+
+bool finite_all(const double *p, size_t n) {
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(p[i])) return false;
+    return true;
+}
+
+int prepare(const pripps_debye_input *in, Prepared &P) {
+    if (!in) return fail(PRIPPS_ERR_USAGE, "input descriptor is NULL");
+    const int64_t n = in->n_atoms;
+    if (n < 0) return fail(PRIPPS_ERR_VALIDATION, "n_atoms must be >= 0");
+    if (n > (int64_t(1) << 30)) return fail(PRIPPS_ERR_VALIDATION, "n_atoms exceeds 2^30");
+    if (in->nq < 0) return fail(PRIPPS_ERR_VALIDATION, "nq must be >= 0");
+    if (in->nq > (1 << 24)) return fail(PRIPPS_ERR_VALIDATION, "nq exceeds 2^24");
+    if (n > 0 && (!in->x || !in->y || !in->z || !in->kind))
+        return fail(PRIPPS_ERR_USAGE, "positions and kind arrays must be non-NULL");
+    if (in->nq > 0 && !in->q) return fail(PRIPPS_ERR_USAGE, "q array is NULL");
+    if (in->n_kinds < 1) return fail(PRIPPS_ERR_VALIDATION, "n_kinds must be >= 1");
+    if (!in->atom_ff && in->nq > 0) return fail(PRIPPS_ERR_USAGE, "atom_ff is NULL");
+    const bool has_w = in->hydration_weight != nullptr, has_wf = in->water_ff != nullptr;
+    if (has_w != has_wf)
+        return fail(PRIPPS_ERR_USAGE,
+                    "hydration needs both water_ff and hydration_weight (or neither)");
+    P.n = (int32_t)n;
+    P.nq = in->nq;
+    P.has_e = in->excluded_ff != nullptr;
+    P.has_h = has_w;
+    const int nq = P.nq;
+    P.q.assign(in->q, in->q + nq);
+    for (int m = 0; m < nq; ++m)
+        if (!std::isfinite(P.q[m]) || P.q[m] < 0.0)
+            return fail(PRIPPS_ERR_VALIDATION, "q values must be finite and >= 0 (q[" +
+                                                   std::to_string(m) + "] = " +
+                                                   std::to_string(P.q[m]) + ")");
+    // used kinds -> dense ids
+    std::vector<int32_t> dense(in->n_kinds, -1);
+    std::vector<int32_t> used;
+    for (int64_t i = 0; i < n; ++i) {
+        int32_t k = in->kind[i];
+        if (k < 0 || k >= in->n_kinds)
+            return fail(PRIPPS_ERR_VALIDATION, "kind id " + std::to_string(k) + " of atom " +
+                                                   std::to_string(i) + " outside [0, n_kinds)");
+        if (dense[k] < 0) {
+            dense[k] = 0;
+            used.push_back(k);
+        }
+    }
+    std::sort(used.begin(), used.end());
+    if ((int)used.size() > kMaxKinds)
+        return fail(PRIPPS_ERR_VALIDATION, "structure uses " + std::to_string(used.size()) +
+                                               " atom kinds; the engine supports at most " +
+                                               std::to_string(kMaxKinds));
+    for (size_t r = 0; r < used.size(); ++r) dense[used[r]] = (int32_t)r;
+    const int K = std::max<int>(1, (int)used.size());
+    P.K = K;
+    P.ffa.assign((size_t)K * nq, 0.0);
+    if (P.has_e) P.ffe.assign((size_t)K * nq, 0.0);
+    for (size_t r = 0; r < used.size(); ++r) {
+        const double *src = in->atom_ff + (size_t)used[r] * nq;
+        if (!finite_all(src, nq))
+            return fail(PRIPPS_ERR_VALIDATION, "atom form factors must be finite");
+        std::copy(src, src + nq, P.ffa.begin() + r * nq);
+        if (P.has_e) {
+            const double *se = in->excluded_ff + (size_t)used[r] * nq;
+            if (!finite_all(se, nq))
+                return fail(PRIPPS_ERR_VALIDATION, "excluded form factors must be finite");
+            std::copy(se, se + nq, P.ffe.begin() + r * nq);
+        }
+    }
+    if (P.has_h) {
+        if (!finite_all(in->water_ff, nq))
+            return fail(PRIPPS_ERR_VALIDATION, "water form factor must be finite");
+        P.ffw.assign(in->water_ff, in->water_ff + nq);
+        if (!finite_all(in->hydration_weight, (size_t)n))
+            return fail(PRIPPS_ERR_VALIDATION, "hydration weights must be finite");
+    }
+    // centroid
+    double cx = 0, cy = 0, cz = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        if (!std::isfinite(in->x[i]) || !std::isfinite(in->y[i]) || !std::isfinite(in->z[i]))
+            return fail(PRIPPS_ERR_VALIDATION, "coordinates must be finite (atom " +
+                                                   std::to_string(i) + ")");
+        cx += in->x[i];
+        cy += in->y[i];
+        cz += in->z[i];
+    }
+    if (n > 0) {
+        cx /= (double)n;
+        cy /= (double)n;
+        cz /= (double)n;
+    }
+    // classes: (dense kind, wet), counting sort keeps the input order inside a class
+    const int C = 2 * K;
+    std::vector<int64_t> count(C + 1, 0);
+    auto cls_of = [&](int64_t i) {
+        const double w = P.has_h ? in->hydration_weight[i] : 0.0;
+        return 2 * dense[in->kind[i]] + (w != 0.0 ? 1 : 0);
+    };
+    for (int64_t i = 0; i < n; ++i) count[cls_of(i) + 1]++;
+    for (int c = 0; c < C; ++c) count[c + 1] += count[c];
+    P.pos.assign((size_t)n * 4, 0.0);
+    P.kind_n.assign(K, 0.0);
+    P.kind_w.assign(K, 0.0);
+    std::vector<int64_t> fillp(count.begin(), count.end() - 1);
+    for (int64_t i = 0; i < n; ++i) {
+        const int c = cls_of(i);
+        const int64_t d = fillp[c]++;
+        const double w = P.has_h ? in->hydration_weight[i] : 0.0;
+        P.pos[4 * d + 0] = in->x[i] - cx;
+        P.pos[4 * d + 1] = in->y[i] - cy;
+        P.pos[4 * d + 2] = in->z[i] - cz;
+        P.pos[4 * d + 3] = w;
+        P.kind_n[c / 2] += 1.0;
+        P.kind_w[c / 2] += w;
+        P.sum_w2 += w * w;
+    }
+    P.n_classes = C;
+    for (int c = 0; c <= C; ++c) P.cls_start[c] = (int32_t)count[c];
+    for (int c = 0; c < C; ++c) {
+        P.cls_kind[c] = (int8_t)(c / 2);
+        P.cls_wet[c] = (int8_t)(c % 2);
+    }
+    // kind-pure i-tiles of up to 64 atoms
+    for (int k = 0; k < K; ++k) {
+        const int32_t b = P.cls_start[2 * k], e = P.cls_start[2 * k + 2];
+        for (int32_t i0 = b; i0 < e; i0 += 64) P.tiles.push_back(make_int4(i0, std::min(64, e - i0), k, 0));
+    }
+    // uniform grid detection (q_m = q0 + m dq to a few ulps)
+    if (nq >= 1) {
+        P.q0 = P.q[0];
+        P.dq = nq >= 2 ? (P.q[nq - 1] - P.q[0]) / (double)(nq - 1) : 0.0;
+        double qmax = 0;
+        for (double v : P.q) qmax = std::max(qmax, std::fabs(v));
+        double dev = 0;
+        for (int m = 0; m < nq; ++m) dev = std::max(dev, std::fabs(P.q[m] - (P.q0 + m * P.dq)));
+        P.uniform = dev <= 64.0 * 2.220446049250313e-16 * std::max(qmax, 1e-300);
+    }
+    P.pair_q_terms = (double)n * (double)(n - 1) / 2.0 * (double)nq;
+    return PRIPPS_OK;
+}
+
+struct Layout {
+    size_t off = 0;
+    size_t take(size_t bytes) {
+        off = (off + 255) & ~size_t(255);
+        size_t at = off;
+        off += bytes;
+        return at;
+    }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// build
+// ---------------------------------------------------------------------------
+static int choose_q(const pripps_tuning *tune, int precision, bool uniform, int nq) {
+    int Q = tune && tune->q_per_lane ? tune->q_per_lane : 0;
+    if (precision == PRIPPS_FP64) {
+        if (Q != 0 && Q != 32 && Q != 64) Q = 0;
+        if (Q == 64 && !uniform) Q = 32;
+        return Q ? Q : 32;
+    }
+    if (Q != 0 && Q != 32 && Q != 64 && Q != 128) Q = 0;
+    if (Q == 128 && !uniform) Q = 64;
+    if (Q) return Q;
+    if (!uniform) return 32;
+    return nq <= 32 ? 32 : 64;
+}
+
+static int build_models(pripps_engine *e, int32_t ns, const pripps_debye_input *inputs,
+                        int32_t precision, const pripps_tuning *tune, pripps_debye_model **out) {
+    const auto t_start = std::chrono::steady_clock::now();
+    if (!e) return fail(PRIPPS_ERR_USAGE, "engine is NULL");
+    if (!out) return fail(PRIPPS_ERR_USAGE, "out is NULL");
+    if (ns < 0) return fail(PRIPPS_ERR_USAGE, "negative structure count");
+    if (precision != PRIPPS_FP32_COMPENSATED && precision != PRIPPS_FP64 &&
+        precision != PRIPPS_FP64_PLAIN)
+        return fail(PRIPPS_ERR_USAGE, "unknown precision " + std::to_string(precision));
+    for (int s = 0; s < ns; ++s) out[s] = nullptr;
+    if (ns == 0) return PRIPPS_OK;
+
+    std::vector<Prepared> P(ns);
+    for (int s = 0; s < ns; ++s) {
+        int rc = prepare(&inputs[s], P[s]);
+        if (rc) {
+            if (ns > 1) g_last_error = "structure " + std::to_string(s) + ": " + g_last_error;
+            return rc;
+        }
+    }
+    const bool fp64 = precision != PRIPPS_FP32_COMPENSATED;
+    const bool plain = precision == PRIPPS_FP64_PLAIN;
+
+    std::lock_guard<std::mutex> lock(e->mu);
+    HIPCHK(hipSetDevice(e->device));
+
+    // models (persistent device memory)
+    std::vector<pripps_debye_model *> models(ns, nullptr);
+    auto cleanup = [&]() {
+        for (auto *m : models) {
+            if (m) {
+                if (m->dev) (void)hipFree(m->dev);
+                delete m;
+            }
+        }
+    };
+    for (int s = 0; s < ns; ++s) {
+        auto *m = new pripps_debye_model();
+        models[s] = m;
+        m->engine = e;
+        m->nq = P[s].nq;
+        m->q = P[s].q;
+        size_t bytes = std::max<size_t>(1, (size_t)11 * P[s].nq) * sizeof(double);
+        hipError_t err = hipMalloc((void **)&m->dev, bytes);
+        if (err != hipSuccess) {
+            cleanup();
+            return hip_fail(err, "hipMalloc(model)");
+        }
+    }
+
+    // ---- plan ----
+    int Q_common = 0;
+    bool uniform_all = true;
+    for (int s = 0; s < ns; ++s) uniform_all = uniform_all && P[s].uniform;
+    std::vector<Item> items;
+    std::vector<int32_t> strip_struct, strip_index, csr_ptr, csr_items;
+    int block = tune && tune->block > 0 ? tune->block : 64;
+    int span = tune && tune->span > 0 ? tune->span : 0;
+    block = std::max(8, std::min(block, 4096));
+    if (!plain) {
+        int maxnq = 0;
+        for (auto &p : P) maxnq = std::max(maxnq, p.nq);
+        Q_common = choose_q(tune, precision, uniform_all, maxnq);
+        // heuristic span: about 16 items per wave slot
+        if (span == 0) {
+            double pairs_tiles = 0;
+            for (auto &p : P)
+                pairs_tiles += (double)p.tiles.size() * 0.5 * p.n * ((p.nq + Q_common - 1) / Q_common);
+            const double slots = std::max(1, e->n_cu) * 8.0;
+            span = (int)(pairs_tiles / (16.0 * slots));
+            span = std::max(256, std::min(8192, (span + 63) / 64 * 64));
+        }
+        span = std::max(64, span);
+        for (int s = 0; s < ns; ++s) {
+            const Prepared &p = P[s];
+            const int nst = (p.nq + Q_common - 1) / Q_common;
+            for (const int4 &t : p.tiles) {
+                for (int32_t j0 = t.x; j0 < p.n; j0 += span) {
+                    const int32_t j1 = std::min<int32_t>(p.n, j0 + span);
+                    for (int st = 0; st < nst; ++st)
+                        items.push_back(Item{s, t.x, t.y, t.z, j0, j1, st, 0});
+                }
+            }
+        }
+        // largest first (stable), so the dynamic queue ends on small items
+        std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) {
+            return (a.j1 - a.j0) > (b.j1 - b.j0);
+        });
+        // CSR of items per (structure, strip)
+        std::vector<int64_t> base(ns + 1, 0);
+        for (int s = 0; s < ns; ++s) base[s + 1] = base[s] + (P[s].nq + Q_common - 1) / Q_common;
+        const int64_t n_strips_total = base[ns];
+        std::vector<int32_t> cnt(n_strips_total + 1, 0);
+        for (const Item &it : items) cnt[base[it.s] + it.strip + 1]++;
+        for (int64_t g = 0; g < n_strips_total; ++g) cnt[g + 1] += cnt[g];
+        csr_ptr.assign(cnt.begin(), cnt.end());
+        csr_items.assign(items.size(), 0);
+        std::vector<int32_t> fillp(cnt.begin(), cnt.end() - 1);
+        for (size_t w = 0; w < items.size(); ++w)
+            csr_items[fillp[base[items[w].s] + items[w].strip]++] = (int32_t)w;
+        strip_struct.resize(n_strips_total);
+        strip_index.resize(n_strips_total);
+        for (int s = 0; s < ns; ++s)
+            for (int64_t st = 0; st < base[s + 1] - base[s]; ++st) {
+                strip_struct[base[s] + st] = s;
+                strip_index[base[s] + st] = (int32_t)st;
+            }
+    }
+
+    // ---- layout of the arena ----
+    Layout L;
+    const size_t atom_bytes = fp64 ? sizeof(Atom<double>) : sizeof(Atom<float>);
+    std::vector<size_t> o_atoms(ns), o_ffa(ns), o_ffe(ns), o_ffw(ns), o_kn(ns), o_kw(ns), o_tiles(ns),
+        o_pp(ns);
+    for (int s = 0; s < ns; ++s) {
+        o_atoms[s] = L.take(std::max<size_t>(1, (size_t)P[s].n) * atom_bytes);
+        o_ffa[s] = L.take(P[s].ffa.size() * 8 + 8);
+        o_ffe[s] = L.take(P[s].ffe.size() * 8 + 8);
+        o_ffw[s] = L.take(P[s].ffw.size() * 8 + 8);
+        o_kn[s] = L.take(P[s].K * 8);
+        o_kw[s] = L.take(P[s].K * 8);
+        if (plain) {
+            o_tiles[s] = L.take(P[s].tiles.size() * sizeof(int4) + 16);
+        }
+    }
+    const size_t o_structs = L.take(ns * sizeof(DevStruct));
+    const size_t o_items = L.take(items.size() * sizeof(Item) + 16);
+    const size_t o_csr_ptr = L.take(csr_ptr.size() * 4 + 4);
+    const size_t o_csr_items = L.take(csr_items.size() * 4 + 4);
+    const size_t o_ss = L.take(strip_struct.size() * 4 + 4);
+    const size_t o_si = L.take(strip_index.size() * 4 + 4);
+    const size_t o_counter = L.take(256);
+    const size_t upload_bytes = L.off;
+    size_t o_partials = 0;
+    size_t partial_bytes = 0;
+    if (plain) {
+        for (int s = 0; s < ns; ++s) {
+            o_pp[s] = L.take(std::max<size_t>(1, P[s].tiles.size() * (size_t)P[s].nq * kProfiles) * 8);
+        }
+    } else {
+        partial_bytes = std::max<size_t>(1, items.size()) * kProfiles * Q_common * 8;
+        o_partials = L.take(partial_bytes);
+    }
+    const size_t total_bytes = L.off;
+
+    int rc = ensure_arena(e, total_bytes);
+    if (!rc) rc = ensure_staging(e, upload_bytes);
+    if (rc) {
+        cleanup();
+        return rc;
+    }
+    char *dbase = (char *)e->arena;
+    char *hbase = (char *)e->staging;
+    std::vector<DevStruct> ds(ns);
+    for (int s = 0; s < ns; ++s) {
+        const Prepared &p = P[s];
+        if (fp64) {
+            std::memcpy(hbase + o_atoms[s], p.pos.data(), p.pos.size() * 8);
+        } else {
+            float *f = (float *)(hbase + o_atoms[s]);
+            for (size_t i = 0; i < p.pos.size(); ++i) f[i] = (float)p.pos[i];
+        }
+        std::memcpy(hbase + o_ffa[s], p.ffa.data(), p.ffa.size() * 8);
+        if (!p.ffe.empty()) std::memcpy(hbase + o_ffe[s], p.ffe.data(), p.ffe.size() * 8);
+        if (!p.ffw.empty()) std::memcpy(hbase + o_ffw[s], p.ffw.data(), p.ffw.size() * 8);
+        std::memcpy(hbase + o_kn[s], p.kind_n.data(), p.K * 8);
+        std::memcpy(hbase + o_kw[s], p.kind_w.data(), p.K * 8);
+        if (plain) std::memcpy(hbase + o_tiles[s], p.tiles.data(), p.tiles.size() * sizeof(int4));
+        DevStruct &d = ds[s];
+        std::memset(&d, 0, sizeof(d));
+        d.atoms = dbase + o_atoms[s];
+        d.q = models[s]->d_q();
+        d.ffa = (const double *)(dbase + o_ffa[s]);
+        d.ffe = p.has_e ? (const double *)(dbase + o_ffe[s]) : nullptr;
+        d.ffw = p.has_h ? (const double *)(dbase + o_ffw[s]) : nullptr;
+        d.profiles = models[s]->d_prof();
+        d.kind_n = (const double *)(dbase + o_kn[s]);
+        d.kind_w = (const double *)(dbase + o_kw[s]);
+        d.sum_w2 = p.sum_w2;
+        d.q0 = p.q0;
+        d.dq = p.dq;
+        d.n = p.n;
+        d.nq = p.nq;
+        d.K = p.K;
+        d.uniform = p.uniform ? 1 : 0;
+        d.n_classes = p.n_classes;
+        d.n_strips = Q_common ? (p.nq + Q_common - 1) / Q_common : 0;
+        for (int c = 0; c <= p.n_classes; ++c) d.cls_start[c] = p.cls_start[c];
+        for (int c = p.n_classes + 1; c <= kMaxClasses; ++c) d.cls_start[c] = p.n;
+        for (int c = 0; c < p.n_classes; ++c) {
+            d.cls_kind[c] = p.cls_kind[c];
+            d.cls_wet[c] = p.cls_wet[c];
+        }
+    }
+    std::memcpy(hbase + o_structs, ds.data(), ns * sizeof(DevStruct));
+    if (!items.empty()) std::memcpy(hbase + o_items, items.data(), items.size() * sizeof(Item));
+    if (!csr_ptr.empty()) std::memcpy(hbase + o_csr_ptr, csr_ptr.data(), csr_ptr.size() * 4);
+    if (!csr_items.empty()) std::memcpy(hbase + o_csr_items, csr_items.data(), csr_items.size() * 4);
+    if (!strip_struct.empty()) std::memcpy(hbase + o_ss, strip_struct.data(), strip_struct.size() * 4);
+    if (!strip_index.empty()) std::memcpy(hbase + o_si, strip_index.data(), strip_index.size() * 4);
+
+    auto run = [&]() -> int {
+        hipStream_t st = e->stream;
+        HIPCHK(hipMemcpyAsync(dbase, hbase, upload_bytes, hipMemcpyHostToDevice, st));
+        for (int s = 0; s < ns; ++s) {
+            if (P[s].nq > 0)
+                HIPCHK(hipMemcpyAsync(models[s]->d_q(), P[s].q.data(), P[s].nq * 8,
+                                      hipMemcpyHostToDevice, st));
+        }
+        HIPCHK(hipEventRecord(e->ev0, st));
+        if (plain) {
+            for (int s = 0; s < ns; ++s) {
+                if (P[s].nq == 0) continue;
+                PlainArgs a;
+                a.sd = (const DevStruct *)(dbase + o_structs) + s;
+                a.tiles = (const int4 *)(dbase + o_tiles[s]);
+                a.n_tiles = (int)P[s].tiles.size();
+                a.partials = (double *)(dbase + o_pp[s]);
+                HIPCHK(launch_plain(a, P[s].nq, st));
+            }
+        } else {
+            int *counter = (int *)(dbase + o_counter);
+            HIPCHK(launch_fill_int(counter, 1, 0, st));
+            if (!items.empty()) {
+                TiledArgs a;
+                a.structs = (const DevStruct *)(dbase + o_structs);
+                a.items = (const Item *)(dbase + o_items);
+                a.n_items = (int64_t)items.size();
+                a.counter = counter;
+                a.partials = (double *)(dbase + o_partials);
+                a.block = block;
+                int bpc = 1;
+                HIPCHK(tiled_occupancy(fp64, Q_common, uniform_all, &bpc));
+                bpc = std::max(1, bpc);
+                const int64_t want = ((int64_t)items.size() + 3) / 4;
+                const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)e->n_cu * bpc));
+                HIPCHK(launch_tiled(fp64, Q_common, uniform_all, a, blocks, st));
+                e->last_tiled = a;
+                e->last_blocks = blocks;
+            }
+            if (!strip_struct.empty()) {
+                FinalizeArgs f;
+                f.structs = (const DevStruct *)(dbase + o_structs);
+                f.csr_ptr = (const int32_t *)(dbase + o_csr_ptr);
+                f.csr_items = (const int32_t *)(dbase + o_csr_items);
+                f.partials = (const double *)(dbase + o_partials);
+                f.strip_struct = (const int32_t *)(dbase + o_ss);
+                f.strip_index = (const int32_t *)(dbase + o_si);
+                f.Q = Q_common;
+                HIPCHK(launch_finalize(f, (int)strip_struct.size(), st));
+                e->last_fin = f;
+            }
+            e->last_counter = counter;
+        }
+        HIPCHK(hipEventRecord(e->ev1, st));
+        HIPCHK(hipStreamSynchronize(st));
+        return PRIPPS_OK;
+    };
+    e->build_serial++;
+    e->last_plain = plain;
+    e->last_fp64 = fp64;
+    e->last_uniform = uniform_all;
+    e->last_Q = Q_common;
+    e->last_items = (int64_t)items.size();
+    e->last_strips = (int)strip_struct.size();
+    e->last_blocks = 0;
+    rc = run();
+    if (rc) {
+        e->build_serial++;  // nothing valid to replay
+        cleanup();
+        return rc;
+    }
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e->ev0, e->ev1);
+    const double total_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    for (int s = 0; s < ns; ++s) {
+        e->refs.fetch_add(1);
+        models[s]->build_serial = e->build_serial;
+        pripps_build_stats &S = models[s]->stats;
+        S.kernel_ms = ms;
+        S.total_ms = total_ms;
+        S.pair_q_terms = P[s].pair_q_terms;
+        S.q_per_lane = Q_common;
+        S.block = block;
+        S.span = span;
+        S.work_items = (int64_t)items.size();
+        out[s] = models[s];
+    }
+    return PRIPPS_OK;
+}
+
+extern "C" int pripps_debye_model_build(pripps_engine *e, const pripps_debye_input *input,
+                                        int32_t precision, const pripps_tuning *tuning,
+                                        pripps_debye_model **out) {
+    if (!out) return fail(PRIPPS_ERR_USAGE, "out is NULL");
+    *out = nullptr;
+    if (!input) return fail(PRIPPS_ERR_USAGE, "input is NULL");
+    return build_models(e, 1, input, precision, tuning, out);
+}
+
+extern "C" int pripps_debye_model_build_batch(pripps_engine *e, int32_t n,
+                                              const pripps_debye_input *inputs, int32_t precision,
+                                              const pripps_tuning *tuning,
+                                              pripps_debye_model **out) {
+    if (n > 0 && !inputs) return fail(PRIPPS_ERR_USAGE, "inputs is NULL");
+    if (precision == PRIPPS_FP64_PLAIN) {
+        // the plain path is single-structure by design: one launch per structure
+        for (int32_t s = 0; s < n; ++s) {
+            int rc = build_models(e, 1, inputs + s, precision, tuning, out + s);
+            if (rc) {
+                for (int32_t t = 0; t < s; ++t) {
+                    pripps_debye_model_free(out[t]);
+                    out[t] = nullptr;
+                }
+                return rc;
+            }
+        }
+        return PRIPPS_OK;
+    }
+    return build_models(e, n, inputs, precision, tuning, out);
+}
+
+// ---------------------------------------------------------------------------
+// model queries
+// ---------------------------------------------------------------------------
+extern "C" int pripps_debye_model_nq(const pripps_debye_model *m, int32_t *nq) {
+    if (!m || !nq) return fail(PRIPPS_ERR_USAGE, "NULL argument");
+    *nq = m->nq;
+    return PRIPPS_OK;
+}
+
+extern "C" int pripps_debye_model_q_values(const pripps_debye_model *m, double *out) {
+    if (!m || (!out && m->nq)) return fail(PRIPPS_ERR_USAGE, "NULL argument");
+    std::copy(m->q.begin(), m->q.end(), out);
+    return PRIPPS_OK;
+}
+
+extern "C" int pripps_debye_model_intensity(const pripps_debye_model *m, double vs, double cd,
+                                            double *total, double *atoms, double *excluded,
+                                            double *hydration) {
+    if (!m) return fail(PRIPPS_ERR_USAGE, "model is NULL");
+    if (m->nq == 0) return PRIPPS_OK;
+    pripps_engine *e = m->engine;
+    std::vector<double> host((size_t)4 * m->nq);
+    {
+        std::lock_guard<std::mutex> lock(e->mu);
+        HIPCHK(hipSetDevice(e->device));
+        HIPCHK(launch_combine(m->d_prof(), m->nq, vs, cd, m->d_out(), e->stream));
+        HIPCHK(hipMemcpyAsync(host.data(), m->d_out(), host.size() * 8, hipMemcpyDeviceToHost,
+                              e->stream));
+        HIPCHK(hipStreamSynchronize(e->stream));
+    }
+    double *dst[4] = {total, atoms, excluded, hydration};
+    for (int k = 0; k < 4; ++k)
+        if (dst[k]) std::copy(host.begin() + (size_t)k * m->nq, host.begin() + (size_t)(k + 1) * m->nq, dst[k]);
+    return PRIPPS_OK;
+}
+
+extern "C" int pripps_debye_model_profiles(const pripps_debye_model *m, double *out) {
+    if (!m || (!out && m->nq)) return fail(PRIPPS_ERR_USAGE, "NULL argument");
+    if (m->nq == 0) return PRIPPS_OK;
+    pripps_engine *e = m->engine;
+    std::lock_guard<std::mutex> lock(e->mu);
+    HIPCHK(hipSetDevice(e->device));
+    HIPCHK(hipMemcpyAsync(out, m->d_prof(), (size_t)6 * m->nq * 8, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return PRIPPS_OK;
+}
+
+extern "C" int pripps_debye_model_fit(const pripps_debye_model *m, const double *q,
+                                      const double *inten, const double *sigma, int64_t n,
+                                      int32_t method, int32_t c1_points, int32_t c2_points,
+                                      const double *c1_bounds, const double *c2_bounds,
+                                      pripps_fit_result *out) {
+    if (!m || !out) return fail(PRIPPS_ERR_USAGE, "NULL argument");
+    if (n <= 0) return fail(PRIPPS_ERR_VALIDATION, "experimental data is empty");
+    if (!q || !inten) return fail(PRIPPS_ERR_USAGE, "q and intensity must be non-NULL");
+    if (method != PRIPPS_FIT_NELDER_MEAD && method != PRIPPS_FIT_GRID)
+        return fail(PRIPPS_ERR_USAGE, "unknown fit method " + std::to_string(method));
+    const double c1d[2] = {0.5, 2.0}, c2d[2] = {0.0, 0.1};
+    const double *c1b = c1_bounds ? c1_bounds : c1d;
+    const double *c2b = c2_bounds ? c2_bounds : c2d;
+    if (method == PRIPPS_FIT_GRID && (c1_points < 2 || c2_points < 2))
+        return fail(PRIPPS_ERR_VALIDATION,
+                    "grid fit requires at least 2 points per axis (got c1_points=" +
+                        std::to_string(c1_points) + ", c2_points=" + std::to_string(c2_points) + ")");
+    const double *bb[2] = {c1b, c2b};
+    const char *nm[2] = {"c1", "c2"};
+    for (int k = 0; k < 2; ++k) {
+        if (!std::isfinite(bb[k][0]) || !std::isfinite(bb[k][1]))
+            return fail(PRIPPS_ERR_VALIDATION, std::string(nm[k]) + " bounds must be finite");
+        if (bb[k][0] >= bb[k][1])
+            return fail(PRIPPS_ERR_VALIDATION,
+                        std::string(nm[k]) + " lower bound must be less than upper bound");
+    }
+    if (m->nq == 0) return fail(PRIPPS_ERR_VALIDATION, "model has no q points");
+    pripps_engine *e = m->engine;
+    std::lock_guard<std::mutex> lock(e->mu);
+    HIPCHK(hipSetDevice(e->device));
+    std::string err;
+    int rc = device_fit(e->stream, m->d_prof(), m->d_q(), m->nq, q, inten, sigma, n, method,
+                        c1_points, c2_points, c1b, c2b, out, &err);
+    if (rc) return fail(rc, err);
+    return PRIPPS_OK;
+}
+
+extern "C" int pripps_debye_model_recompute(pripps_debye_model *m, int32_t repeats,
+                                            double *pair_ms_mean, double *total_ms) {
+    if (!m) return fail(PRIPPS_ERR_USAGE, "model is NULL");
+    if (repeats < 0) return fail(PRIPPS_ERR_USAGE, "repeats must be >= 0");
+    pripps_engine *e = m->engine;
+    std::lock_guard<std::mutex> lock(e->mu);
+    if (m->build_serial == 0 || m->build_serial != e->build_serial || e->last_plain)
+        return fail(PRIPPS_ERR_USAGE,
+                    "recompute needs a tiled model from the engine's most recent build");
+    HIPCHK(hipSetDevice(e->device));
+    hipStream_t st = e->stream;
+    std::vector<hipEvent_t> ev(2 * (size_t)repeats + 2, nullptr);
+    for (auto &x : ev) HIPCHK(hipEventCreate(&x));
+    auto destroy = [&]() {
+        for (auto &x : ev)
+            if (x) (void)hipEventDestroy(x);
+    };
+    hipError_t err = hipEventRecord(ev[0], st);
+    for (int r = 0; r < repeats && err == hipSuccess; ++r) {
+        err = launch_fill_int(e->last_counter, 1, 0, st);
+        if (err == hipSuccess) err = hipEventRecord(ev[2 + 2 * r], st);
+        if (err == hipSuccess && e->last_items > 0)
+            err = launch_tiled(e->last_fp64, e->last_Q, e->last_uniform, e->last_tiled,
+                               e->last_blocks, st);
+        if (err == hipSuccess) err = hipEventRecord(ev[3 + 2 * r], st);
+        if (err == hipSuccess && e->last_strips > 0)
+            err = launch_finalize(e->last_fin, e->last_strips, st);
+    }
+    if (err == hipSuccess) err = hipEventRecord(ev[1], st);
+    if (err == hipSuccess) err = hipStreamSynchronize(st);
+    if (err != hipSuccess) {
+        destroy();
+        return hip_fail(err, "pripps_debye_model_recompute");
+    }
+    double sum = 0;
+    for (int r = 0; r < repeats; ++r) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, ev[2 + 2 * r], ev[3 + 2 * r]);
+        sum += ms;
+    }
+    float tot = 0;
+    (void)hipEventElapsedTime(&tot, ev[0], ev[1]);
+    destroy();
+    if (pair_ms_mean) *pair_ms_mean = repeats ? sum / repeats : 0.0;
+    if (total_ms) *total_ms = tot;
+    return PRIPPS_OK;
+}
+
+extern "C" int pripps_debye_model_from_profiles(pripps_engine *e, const double *q, int32_t nq,
+                                                const double *profiles,
+                                                pripps_debye_model **out) {
+    if (!e || !out) return fail(PRIPPS_ERR_USAGE, "NULL argument");
+    *out = nullptr;
+    if (nq < 0) return fail(PRIPPS_ERR_VALIDATION, "nq must be >= 0");
+    if (nq > 0 && (!q || !profiles)) return fail(PRIPPS_ERR_USAGE, "q and profiles must be non-NULL");
+    if (!finite_all(q, nq) || !finite_all(profiles, (size_t)6 * nq))
+        return fail(PRIPPS_ERR_VALIDATION, "q and profiles must be finite");
+    auto *m = new pripps_debye_model();
+    m->engine = e;
+    m->nq = nq;
+    m->q.assign(q, q + nq);
+    {
+        std::lock_guard<std::mutex> lock(e->mu);
+        hipError_t err = hipSetDevice(e->device);
+        if (err == hipSuccess)
+            err = hipMalloc((void **)&m->dev, std::max<size_t>(1, (size_t)11 * nq) * sizeof(double));
+        if (err == hipSuccess && nq > 0)
+            err = hipMemcpyAsync(m->d_prof(), profiles, (size_t)6 * nq * 8, hipMemcpyHostToDevice, e->stream);
+        if (err == hipSuccess && nq > 0)
+            err = hipMemcpyAsync(m->d_q(), q, (size_t)nq * 8, hipMemcpyHostToDevice, e->stream);
+        if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+        if (err != hipSuccess) {
+            if (m->dev) (void)hipFree(m->dev);
+            delete m;
+            return hip_fail(err, "pripps_debye_model_from_profiles");
+        }
+    }
+    e->refs.fetch_add(1);
+    *out = m;
+    return PRIPPS_OK;
+}
+
+extern "C" int pripps_debye_model_stats(const pripps_debye_model *m, pripps_build_stats *out) {
+    if (!m || !out) return fail(PRIPPS_ERR_USAGE, "NULL argument");
+    *out = m->stats;
+    return PRIPPS_OK;
+}
+
+extern "C" int pripps_debye_model_free(pripps_debye_model *m) {
+    if (!m) return PRIPPS_OK;
+    pripps_engine *e = m->engine;
+    if (m->dev) {
+        (void)hipSetDevice(e->device);
+        (void)hipFree(m->dev);
+    }
+    delete m;
+    engine_release(e);
+    return PRIPPS_OK;
+}
+
+namespace pripps {
+// used by the serving loop
+int engine_build(pripps_engine *e, int32_t n, const pripps_debye_input *inputs, int32_t precision,
+                 const pripps_tuning *tuning, pripps_debye_model **out) {
+    return build_models(e, n, inputs, precision, tuning, out);
+}
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+const std::string &get_last_error() { return g_last_error; }
+}  // namespace pripps

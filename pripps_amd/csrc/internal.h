@@ -1,0 +1,94 @@
+// Internal layouts shared by the HIP kernels and the host engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pripps {
+
+// Maximum number of distinct atom kinds in one structure; classes are
+// (kind, hydrated) pairs, so there are at most 2 * kMaxKinds of them.
+constexpr int kMaxKinds = 16;
+constexpr int kMaxClasses = 2 * kMaxKinds;
+constexpr int kProfiles = 6;  // DebyeTerms order: AA EE HH AE AH EH
+
+// One atom of the sorted SoA-in-AoS stream: centred coordinates and the
+// hydration weight, 16 bytes (FP32) or 32 bytes (FP64) per atom so a wave
+// fetches one j atom with a single scalar load.
+template <typename T>
+struct alignas(4 * sizeof(T)) Atom {
+    T x, y, z, w;
+};
+
+// Per-structure description, resident in device memory.
+struct DevStruct {
+    const void *atoms;      // Atom<T>[n] sorted by class
+    const double *q;        // [nq] q values
+    const double *ffa;      // [K][nq] atom form factors
+    const double *ffe;      // [K][nq] excluded-volume form factors or nullptr
+    const double *ffw;      // [nq] water form factor or nullptr
+    double *profiles;       // [6][nq] output
+    const double *kind_n;   // [K] atoms per kind
+    const double *kind_w;   // [K] sum of hydration weights per kind
+    double sum_w2;          // sum of squared hydration weights
+    double q0, dq;          // uniform grid: q_m = q0 + m dq
+    int32_t n, nq, K, uniform;
+    int32_t n_classes;
+    int32_t n_strips;       // ceil(nq / Q)
+    int64_t item_begin;     // first item of this structure (items are grouped
+    int64_t csr_begin;      //   per structure); CSR of items per strip
+    int32_t cls_start[kMaxClasses + 1];
+    int8_t cls_kind[kMaxClasses];
+    int8_t cls_wet[kMaxClasses];
+};
+
+// One wave's work: a 64-atom i-tile of one kind against a j-span, for one
+// q strip of Q points.
+struct Item {
+    int32_t s;       // structure index
+    int32_t i0;      // first (sorted) atom of the i-tile
+    int32_t icount;  // atoms in the tile, 1..64
+    int32_t ikind;   // kind of every atom in the tile
+    int32_t j0, j1;  // j span [j0, j1), j0 >= i0
+    int32_t strip;   // q strip index
+    int32_t pad;
+};
+
+// Kernel argument blocks.
+struct TiledArgs {
+    const DevStruct *structs;
+    const Item *items;
+    int64_t n_items;
+    int *counter;      // work-queue head, zeroed before launch
+    double *partials;  // [n_items][6][Q]
+    int block;         // pairs summed in FP32 before the FP64 carry
+};
+
+struct FinalizeArgs {
+    const DevStruct *structs;
+    const int32_t *csr_ptr;       // per (structure, strip): offsets into csr_items
+    const int32_t *csr_items;     // item ids
+    const double *partials;
+    const int32_t *strip_struct;  // flattened (structure, strip) -> structure
+    const int32_t *strip_index;   // flattened (structure, strip) -> strip
+    int Q;
+};
+
+struct PlainArgs {
+    const DevStruct *sd;
+    const int4 *tiles;  // (i0, icount, ikind, 0)
+    int n_tiles;
+    double *partials;   // [n_tiles][nq][6]
+};
+
+// launch wrappers (debye_kernels.hip)
+hipError_t launch_tiled(bool fp64, int Q, bool uniform, const TiledArgs &a, int blocks,
+                        hipStream_t st);
+hipError_t tiled_occupancy(bool fp64, int Q, bool uniform, int *blocks_per_cu);
+hipError_t launch_finalize(const FinalizeArgs &a, int n_strips_total, hipStream_t st);
+hipError_t launch_plain(const PlainArgs &a, int nq, hipStream_t st);
+hipError_t launch_fill_int(int *p, int n, int v, hipStream_t st);
+hipError_t launch_combine(const double *prof, int nq, double vs, double cd, double *out,
+                          hipStream_t st);
+
+}  // namespace pripps

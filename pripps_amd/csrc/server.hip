@@ -1,0 +1,289 @@
+// Serving loop for Debye solves: continuous batching with admission by free
+// device memory.
+//
+// Requests are copied at submit time and queued.  Two worker lanes, each with
+// its own engine (stream + device arena) on the same GPU, repeatedly take the
+// oldest queued requests whose estimated device footprint fits what is left
+// of the memory budget, and build them as one batched persistent launch.  A
+// lane that is building leaves the queue to the other lane, so requests that
+// arrive while a batch is on the device start in the next batch and run
+// alongside it (the second kernel's waves take CUs as the first one's drain).
+// Finished models keep their profiles on the device and are handed to the
+// caller by pripps_server_wait together with the request's latency (submit to
+// completion).
+//
+// The reference has no serving layer; this loop wraps the build that replaces
+// debye::build_model (src/debye.rs:78-136).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "internal.h"
+#include "pripps_debye.h"
+
+namespace pripps {
+int engine_build(pripps_engine *e, int32_t n, const pripps_debye_input *inputs, int32_t precision,
+                 const pripps_tuning *tuning, pripps_debye_model **out);
+void set_last_error(const std::string &msg);
+const std::string &get_last_error();
+}  // namespace pripps
+
+using namespace pripps;
+using Clock = std::chrono::steady_clock;
+
+namespace {
+
+struct Request {
+    int64_t id = 0;
+    Clock::time_point t_submit;
+    // owned copies of the caller's arrays
+    std::vector<double> x, y, z, q, ffa, ffe, ffw, w;
+    std::vector<int32_t> kind;
+    pripps_debye_input in{};
+    uint64_t bytes = 0;
+    // completion
+    bool done = false;
+    int status = PRIPPS_OK;
+    std::string error;
+    pripps_debye_model *model = nullptr;
+    double latency_ms = 0;
+};
+
+uint64_t estimate_bytes(const pripps_debye_input &in) {
+    // model (11 nq doubles) + sorted atoms + tables + a share of the partial
+    // profile buffer (one 6 x 64 FP64 slot per 64 x 512 pair tile and strip)
+    const double n = (double)in.n_atoms, nq = (double)in.nq;
+    const double tiles = n / 64.0 + 16.0;
+    const double spans = n / 512.0 + 1.0;
+    const double strips = nq / 64.0 + 1.0;
+    const double partial = tiles * spans * strips * 6.0 * 64.0 * 8.0 * 0.5;
+    return (uint64_t)(11.0 * nq * 8.0 + n * 40.0 + 2.0 * 16.0 * nq * 8.0 + partial + 4096.0);
+}
+
+}  // namespace
+
+struct pripps_server {
+    int device = 0;
+    pripps_server_config cfg{};
+    std::vector<pripps_engine *> lanes;
+    std::vector<std::thread> workers;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request *> queue;
+    std::unordered_map<int64_t, Request *> all;
+    int64_t next_id = 1;
+    uint64_t in_use = 0;  // estimated bytes of batches on the device
+    bool stopping = false;
+
+    void worker(pripps_engine *lane) {
+        for (;;) {
+            std::vector<Request *> batch;
+            uint64_t batch_bytes = 0;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv_work.wait(lk, [&] { return stopping || !queue.empty(); });
+                if (stopping && queue.empty()) return;
+                uint64_t free_b = 0, total_b = 0;
+                lk.unlock();
+                pripps_engine_memory(lane, &free_b, &total_b);
+                lk.lock();
+                uint64_t budget = cfg.memory_budget ? cfg.memory_budget : (uint64_t)(0.9 * (double)free_b);
+                const int cap = cfg.max_in_flight > 0 ? cfg.max_in_flight : 256;
+                while (!queue.empty() && (int)batch.size() < cap) {
+                    Request *r = queue.front();
+                    if (r->bytes > budget) {
+                        // can never fit: fail it rather than stall the queue
+                        queue.pop_front();
+                        r->done = true;
+                        r->status = PRIPPS_ERR_OUT_OF_MEMORY;
+                        r->error = "request needs about " + std::to_string(r->bytes) +
+                                   " bytes of device memory; the serving budget is " +
+                                   std::to_string(budget);
+                        r->latency_ms = std::chrono::duration<double, std::milli>(Clock::now() - r->t_submit).count();
+                        cv_done.notify_all();
+                        continue;
+                    }
+                    if (in_use + batch_bytes + r->bytes > budget) break;  // wait for memory
+                    queue.pop_front();
+                    batch.push_back(r);
+                    batch_bytes += r->bytes;
+                }
+                if (batch.empty()) {
+                    // the head does not fit while other batches hold memory
+                    cv_work.wait_for(lk, std::chrono::milliseconds(1));
+                    continue;
+                }
+                in_use += batch_bytes;
+            }
+            std::vector<pripps_debye_input> ins(batch.size());
+            for (size_t k = 0; k < batch.size(); ++k) ins[k] = batch[k]->in;
+            std::vector<pripps_debye_model *> out(batch.size(), nullptr);
+            const int rc = engine_build(lane, (int32_t)batch.size(), ins.data(), cfg.precision,
+                                        nullptr, out.data());
+            const std::string err = rc ? get_last_error() : std::string();
+            const auto now = Clock::now();
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                in_use -= batch_bytes;
+                for (size_t k = 0; k < batch.size(); ++k) {
+                    Request *r = batch[k];
+                    r->done = true;
+                    r->status = rc;
+                    r->error = err;
+                    r->model = out[k];
+                    r->latency_ms = std::chrono::duration<double, std::milli>(now - r->t_submit).count();
+                    // inputs are no longer needed
+                    r->x.clear(); r->x.shrink_to_fit();
+                    r->y.clear(); r->y.shrink_to_fit();
+                    r->z.clear(); r->z.shrink_to_fit();
+                    r->kind.clear(); r->kind.shrink_to_fit();
+                    r->w.clear(); r->w.shrink_to_fit();
+                }
+            }
+            cv_done.notify_all();
+            cv_work.notify_all();
+        }
+    }
+};
+
+extern "C" int pripps_server_create(pripps_engine *engine, const pripps_server_config *config,
+                                    pripps_server **out) {
+    if (!engine || !out) {
+        set_last_error("pripps_server_create: NULL argument");
+        return PRIPPS_ERR_USAGE;
+    }
+    *out = nullptr;
+    uint64_t f = 0, t = 0;
+    int rc = pripps_engine_memory(engine, &f, &t);
+    if (rc) return rc;
+    auto *s = new pripps_server();
+    if (config) s->cfg = *config;
+    if (s->cfg.precision != PRIPPS_FP32_COMPENSATED && s->cfg.precision != PRIPPS_FP64) {
+        delete s;
+        set_last_error("pripps_server_create: precision must be FP32_COMPENSATED or FP64");
+        return PRIPPS_ERR_USAGE;
+    }
+    // pripps_engine_memory made the engine's device current
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    s->device = dev;
+    for (int k = 0; k < 2; ++k) {
+        pripps_engine *lane = nullptr;
+        rc = pripps_engine_create(dev, &lane);
+        if (rc) {
+            for (auto *l : s->lanes) pripps_engine_destroy(l);
+            delete s;
+            return rc;
+        }
+        s->lanes.push_back(lane);
+    }
+    for (auto *lane : s->lanes) s->workers.emplace_back([s, lane] { s->worker(lane); });
+    *out = s;
+    return PRIPPS_OK;
+}
+
+extern "C" int pripps_server_submit(pripps_server *s, const pripps_debye_input *in,
+                                    int64_t *request_id) {
+    if (!s || !in || !request_id) {
+        set_last_error("pripps_server_submit: NULL argument");
+        return PRIPPS_ERR_USAGE;
+    }
+    if (in->n_atoms < 0 || in->nq < 0 || in->n_kinds < 1 ||
+        (in->n_atoms > 0 && (!in->x || !in->y || !in->z || !in->kind)) ||
+        (in->nq > 0 && (!in->q || !in->atom_ff)) ||
+        ((in->water_ff != nullptr) != (in->hydration_weight != nullptr))) {
+        set_last_error("pripps_server_submit: invalid input descriptor");
+        return PRIPPS_ERR_USAGE;
+    }
+    auto *r = new Request();
+    const size_t n = (size_t)in->n_atoms, nq = (size_t)in->nq, kq = (size_t)in->n_kinds * nq;
+    r->x.assign(in->x, in->x + n);
+    r->y.assign(in->y, in->y + n);
+    r->z.assign(in->z, in->z + n);
+    r->kind.assign(in->kind, in->kind + n);
+    r->q.assign(in->q, in->q + nq);
+    r->ffa.assign(in->atom_ff, in->atom_ff + kq);
+    if (in->excluded_ff) r->ffe.assign(in->excluded_ff, in->excluded_ff + kq);
+    if (in->water_ff) {
+        r->ffw.assign(in->water_ff, in->water_ff + nq);
+        r->w.assign(in->hydration_weight, in->hydration_weight + n);
+    }
+    r->in = *in;
+    r->in.x = r->x.data();
+    r->in.y = r->y.data();
+    r->in.z = r->z.data();
+    r->in.kind = r->kind.data();
+    r->in.q = r->q.data();
+    r->in.atom_ff = r->ffa.data();
+    r->in.excluded_ff = in->excluded_ff ? r->ffe.data() : nullptr;
+    r->in.water_ff = in->water_ff ? r->ffw.data() : nullptr;
+    r->in.hydration_weight = in->water_ff ? r->w.data() : nullptr;
+    r->bytes = estimate_bytes(*in);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        r->id = s->next_id++;
+        r->t_submit = Clock::now();
+        s->all[r->id] = r;
+        s->queue.push_back(r);
+        *request_id = r->id;
+    }
+    s->cv_work.notify_one();
+    return PRIPPS_OK;
+}
+
+extern "C" int pripps_server_wait(pripps_server *s, int64_t id, int32_t timeout_ms,
+                                  pripps_debye_model **model, double *latency_ms) {
+    if (!s || !model) {
+        set_last_error("pripps_server_wait: NULL argument");
+        return PRIPPS_ERR_USAGE;
+    }
+    *model = nullptr;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->all.find(id);
+    if (it == s->all.end()) {
+        set_last_error("pripps_server_wait: unknown request id " + std::to_string(id));
+        return PRIPPS_ERR_USAGE;
+    }
+    Request *r = it->second;
+    auto ready = [&] { return r->done; };
+    if (timeout_ms < 0) s->cv_done.wait(lk, ready);
+    else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready)) {
+        set_last_error("request " + std::to_string(id) + " still running");
+        return PRIPPS_ERR_NOT_READY;
+    }
+    s->all.erase(it);
+    const int status = r->status;
+    const std::string err = r->error;
+    *model = r->model;
+    if (latency_ms) *latency_ms = r->latency_ms;
+    delete r;
+    lk.unlock();
+    if (status) set_last_error(err);
+    return status;
+}
+
+extern "C" int pripps_server_destroy(pripps_server *s) {
+    if (!s) return PRIPPS_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stopping = true;
+    }
+    s->cv_work.notify_all();
+    for (auto &th : s->workers) th.join();
+    for (auto &kv : s->all) {
+        if (kv.second->model) pripps_debye_model_free(kv.second->model);
+        delete kv.second;
+    }
+    for (auto *l : s->lanes) pripps_engine_destroy(l);
+    delete s;
+    return PRIPPS_OK;
+}
