@@ -153,8 +153,26 @@ struct Seed32 {
     float s, d, lam;
 };
 
+// sin(2 pi x) for |x| <= 1/8 revolution as an odd polynomial in h = 2 pi x,
+// accurate to a few ulps *relative*: the recurrence constants
+// lambda = 4 sin^2(h) and delta_1 = 2 cos(a + h) sin(h) / r need sin(h) with
+// full relative accuracy, which the hardware sine (absolute accuracy) does not
+// give for the tiny h = dq r / 2 of fine q grids.  Remainder < h^11/11! <
+// 2e-10 relative at the interval end.
+__device__ __forceinline__ float sin_small_rev(float x) {
+    const float h = 6.28318530717958647692f * x;
+    const float h2 = h * h;
+    float p = 2.75573192e-6f;            //  1/9!
+    p = p * h2 - 1.98412698e-4f;         // -1/7!
+    p = p * h2 + 8.33333333e-3f;         //  1/5!
+    p = p * h2 - 1.66666667e-1f;         // -1/3!
+    return h + h * (h2 * p);
+}
+
 // FP32 seed: q0rev = q_s / 2pi, hrev = dq / 4pi (revolutions per Angstrom).
-__device__ __forceinline__ Seed32 seed_f32(float r2, float q0rev, float hrev, float take) {
+// small_h: the host guarantees hrev * r <= 1/8 for every pair of the build.
+__device__ __forceinline__ Seed32 seed_f32(float r2, float q0rev, float hrev, float take,
+                                           bool small_h) {
     r2 = fmaxf(r2, 1e-12f);  // coincident atoms: the r -> 0 limit s_k -> q_k
     const float rinv = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * rinv;
@@ -162,7 +180,7 @@ __device__ __forceinline__ Seed32 seed_f32(float r2, float q0rev, float hrev, fl
     const float hr = hrev * r;
     const float sa = __builtin_amdgcn_sinf(__builtin_amdgcn_fractf(arev));
     const float cah = __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(arev + hr));
-    const float sh = __builtin_amdgcn_sinf(hr);
+    const float sh = small_h ? sin_small_rev(hr) : __builtin_amdgcn_sinf(__builtin_amdgcn_fractf(hr));
     const float amp = take * rinv;
     Seed32 z;
     z.s = sa * amp;
@@ -197,8 +215,8 @@ __device__ __forceinline__ Seed64 seed_f64(double r2, double q0, double hdq, dou
 template <typename T, int Q, bool UNIFORM, bool WET, bool MASK>
 __device__ __forceinline__ void pair_block(const Atom<T> *__restrict__ atoms, int jb, int je,
                                            const Atom<T> &ai, int iidx, T lane_take,
-                                           const T (&qv)[Q], T q0p, T hp, T (&acc0)[Q],
-                                           T (&accw)[Q]) {
+                                           const T (&qv)[Q], T q0p, T hp, bool small_h,
+                                           T (&acc0)[Q], T (&accw)[Q]) {
     const PRIPPS_CONST Atom<T> *ca = as_const(atoms);
     Atom<T> next = load_atom(ca, jb);
     for (int j = jb; j < je; ++j) {
@@ -211,7 +229,7 @@ __device__ __forceinline__ void pair_block(const Atom<T> *__restrict__ atoms, in
         if constexpr (UNIFORM) {
             T s, d, lam;
             if constexpr (sizeof(T) == 4) {
-                Seed32 z = seed_f32(r2, q0p, hp, take);
+                Seed32 z = seed_f32(r2, q0p, hp, take, small_h);
                 s = z.s;
                 d = z.d;
                 lam = z.lam;
@@ -280,6 +298,7 @@ __global__ void __launch_bounds__(256) debye_tiled_kernel(TiledArgs args) {
         const bool wet_i = __ballot(wi != T(0)) != 0;
 
         const int qbase = it.strip * Q;
+        const bool small_h = sd.small_h != 0;
         // strip constants
         T qv[Q];
         T q0p = T(0), hp = T(0);
@@ -329,11 +348,11 @@ __global__ void __launch_bounds__(256) debye_tiled_kernel(TiledArgs args) {
             for (int k = 0; k < Q; ++k) acc0[k] = accw[k] = T(0);
 
             if (wet) {
-                if (mask) pair_block<T, Q, UNIFORM, true, true>(atoms, j, jend, ai, iidx, lane_take, qv, q0p, hp, acc0, accw);
-                else pair_block<T, Q, UNIFORM, true, false>(atoms, j, jend, ai, iidx, lane_take, qv, q0p, hp, acc0, accw);
+                if (mask) pair_block<T, Q, UNIFORM, true, true>(atoms, j, jend, ai, iidx, lane_take, qv, q0p, hp, small_h, acc0, accw);
+                else pair_block<T, Q, UNIFORM, true, false>(atoms, j, jend, ai, iidx, lane_take, qv, q0p, hp, small_h, acc0, accw);
             } else {
-                if (mask) pair_block<T, Q, UNIFORM, false, true>(atoms, j, jend, ai, iidx, lane_take, qv, q0p, hp, acc0, accw);
-                else pair_block<T, Q, UNIFORM, false, false>(atoms, j, jend, ai, iidx, lane_take, qv, q0p, hp, acc0, accw);
+                if (mask) pair_block<T, Q, UNIFORM, false, true>(atoms, j, jend, ai, iidx, lane_take, qv, q0p, hp, small_h, acc0, accw);
+                else pair_block<T, Q, UNIFORM, false, false>(atoms, j, jend, ai, iidx, lane_take, qv, q0p, hp, small_h, acc0, accw);
             }
 
             // FP64 carry: wave sums of S = sum s, Si = sum w_i s, Sw = sum w_j s,
@@ -417,17 +436,23 @@ __device__ void diagonal_and_zero(const PRIPPS_CONST DevStruct &sd, int m, doubl
     zero[5] = se * sh;
 }
 
-__global__ void __launch_bounds__(256) debye_finalize_kernel(FinalizeArgs args) {
+// Stage 1: block (strip g, chunk c) sums the partials of items
+// csr[g][c*kFinChunk .. (c+1)*kFinChunk) into stage[g][c][6][Q].  Thread t
+// takes q-point t % Q and every G-th item of the chunk (G = 256 / Q); the G
+// group sums are combined in LDS in group order, so the result is fixed by
+// the plan alone.
+__global__ void __launch_bounds__(256) debye_reduce_kernel(FinalizeArgs args) {
     __shared__ double red[256][kProfiles];
-    const int g_strip = blockIdx.x;
-    const int s = args.strip_struct[g_strip];
-    const int strip = args.strip_index[g_strip];
-    const PRIPPS_CONST DevStruct &sd = as_const(args.structs)[s];
+    const int g_strip = blockIdx.y;
+    const int chunk = blockIdx.x;
     const int Q = args.Q;
     const int G = 256 / Q;
     const int ql = threadIdx.x % Q;
     const int g = threadIdx.x / Q;
-    const int b = args.csr_ptr[g_strip], e = args.csr_ptr[g_strip + 1];
+    const int b = args.csr_ptr[g_strip] + chunk * kFinChunk;
+    const int e = min(args.csr_ptr[g_strip + 1], b + kFinChunk);
+    const int nchunks = (args.csr_ptr[g_strip + 1] - args.csr_ptr[g_strip] + kFinChunk - 1) / kFinChunk;
+    if (chunk >= nchunks) return;  // uniform for the whole block
     double acc[kProfiles] = {0, 0, 0, 0, 0, 0};
     if (g < G) {
         for (int idx = b + g; idx < e; idx += G) {
@@ -443,18 +468,36 @@ __global__ void __launch_bounds__(256) debye_finalize_kernel(FinalizeArgs args) 
         for (int gg = 1; gg < G; ++gg)
 #pragma unroll
             for (int k = 0; k < kProfiles; ++k) acc[k] += red[gg * Q + ql][k];
-        const int m = strip * Q + ql;
-        if (m < sd.nq) {
-            double diag[kProfiles], zero[kProfiles];
-            diagonal_and_zero(sd, m, diag, zero);
-            const double q = sd.q[m];
+        double *o = args.stage + ((int64_t)args.stage_ptr[g_strip] + chunk) * (kProfiles * Q);
 #pragma unroll
-            for (int k = 0; k < kProfiles; ++k) {
-                const double v = q > 0.0 ? acc[k] / q + diag[k] : zero[k];
-                sd.profiles[k * sd.nq + m] = v;
-            }
-        }
+        for (int k = 0; k < kProfiles; ++k) o[k * Q + ql] = acc[k];
     }
+}
+
+// Stage 2: one thread per q-point of a strip sums the chunk totals in chunk
+// order, divides by q and adds the diagonal (q = 0: closed form).
+__global__ void __launch_bounds__(128) debye_finalize_kernel(FinalizeArgs args) {
+    const int g_strip = blockIdx.x;
+    const int ql = threadIdx.x;
+    const int Q = args.Q;
+    if (ql >= Q) return;
+    const int s = args.strip_struct[g_strip];
+    const int strip = args.strip_index[g_strip];
+    const PRIPPS_CONST DevStruct &sd = as_const(args.structs)[s];
+    const int m = strip * Q + ql;
+    if (m >= sd.nq) return;
+    const int nchunks = args.stage_ptr[g_strip + 1] - args.stage_ptr[g_strip];
+    double acc[kProfiles] = {0, 0, 0, 0, 0, 0};
+    for (int c = 0; c < nchunks; ++c) {
+        const double *p = args.stage + ((int64_t)args.stage_ptr[g_strip] + c) * (kProfiles * Q);
+#pragma unroll
+        for (int k = 0; k < kProfiles; ++k) acc[k] += p[k * Q + ql];
+    }
+    double diag[kProfiles], zero[kProfiles];
+    diagonal_and_zero(sd, m, diag, zero);
+    const double q = sd.q[m];
+#pragma unroll
+    for (int k = 0; k < kProfiles; ++k) sd.profiles[k * sd.nq + m] = q > 0.0 ? acc[k] / q + diag[k] : zero[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -611,7 +654,12 @@ hipError_t tiled_occupancy(bool fp64, int Q, bool uniform, int *blocks_per_cu) {
 }
 
 hipError_t launch_finalize(const FinalizeArgs &a, int n_strips_total, hipStream_t st) {
-    hipLaunchKernelGGL(debye_finalize_kernel, dim3(n_strips_total), dim3(256), 0, st, a);
+    if (a.max_chunks > 0) {
+        hipLaunchKernelGGL(debye_reduce_kernel, dim3(a.max_chunks, n_strips_total), dim3(256), 0, st, a);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(debye_finalize_kernel, dim3(n_strips_total), dim3(128), 0, st, a);
     return hipGetLastError();
 }
 

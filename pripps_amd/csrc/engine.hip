@@ -185,6 +185,7 @@ struct Prepared {
     int32_t n = 0, nq = 0, K = 0;
     bool has_e = false, has_h = false;
     bool uniform = false;
+    bool small_h = false;   // dq * r_max / (4 pi) <= 1/8
     double q0 = 0, dq = 0;
     std::vector<double> q;
     std::vector<double> pos;  // [n][4] centred x y z w, sorted by class
@@ -334,6 +335,14 @@ int prepare(const pripps_debye_input *in, Prepared &P) {
         for (int m = 0; m < nq; ++m) dev = std::max(dev, std::fabs(P.q[m] - (P.q0 + m * P.dq)));
         P.uniform = dev <= 64.0 * 2.220446049250313e-16 * std::max(qmax, 1e-300);
     }
+    // largest pair distance is at most twice the largest distance from the centroid
+    double rmax2 = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double *p4 = &P.pos[4 * i];
+        rmax2 = std::max(rmax2, p4[0] * p4[0] + p4[1] * p4[1] + p4[2] * p4[2]);
+    }
+    const double rmax = 2.0 * std::sqrt(rmax2) * (1.0 + 1e-6) + 1e-6;
+    P.small_h = std::fabs(P.dq) * rmax / (4.0 * M_PI) <= 0.125;
     P.pair_q_terms = (double)n * (double)(n - 1) / 2.0 * (double)nq;
     return PRIPPS_OK;
 }
@@ -422,7 +431,8 @@ static int build_models(pripps_engine *e, int32_t ns, const pripps_debye_input *
     bool uniform_all = true;
     for (int s = 0; s < ns; ++s) uniform_all = uniform_all && P[s].uniform;
     std::vector<Item> items;
-    std::vector<int32_t> strip_struct, strip_index, csr_ptr, csr_items;
+    std::vector<int32_t> strip_struct, strip_index, csr_ptr, csr_items, stage_ptr;
+    int max_chunks = 0;
     int block = tune && tune->block > 0 ? tune->block : 64;
     int span = tune && tune->span > 0 ? tune->span : 0;
     block = std::max(8, std::min(block, 4096));
@@ -467,6 +477,12 @@ static int build_models(pripps_engine *e, int32_t ns, const pripps_debye_input *
         std::vector<int32_t> fillp(cnt.begin(), cnt.end() - 1);
         for (size_t w = 0; w < items.size(); ++w)
             csr_items[fillp[base[items[w].s] + items[w].strip]++] = (int32_t)w;
+        stage_ptr.assign(n_strips_total + 1, 0);
+        for (int64_t g = 0; g < n_strips_total; ++g) {
+            const int nch = (csr_ptr[g + 1] - csr_ptr[g] + kFinChunk - 1) / kFinChunk;
+            stage_ptr[g + 1] = stage_ptr[g] + nch;
+            max_chunks = std::max(max_chunks, nch);
+        }
         strip_struct.resize(n_strips_total);
         strip_index.resize(n_strips_total);
         for (int s = 0; s < ns; ++s)
@@ -498,9 +514,10 @@ static int build_models(pripps_engine *e, int32_t ns, const pripps_debye_input *
     const size_t o_csr_items = L.take(csr_items.size() * 4 + 4);
     const size_t o_ss = L.take(strip_struct.size() * 4 + 4);
     const size_t o_si = L.take(strip_index.size() * 4 + 4);
+    const size_t o_sp = L.take(stage_ptr.size() * 4 + 4);
     const size_t o_counter = L.take(256);
     const size_t upload_bytes = L.off;
-    size_t o_partials = 0;
+    size_t o_partials = 0, o_stage = 0;
     size_t partial_bytes = 0;
     if (plain) {
         for (int s = 0; s < ns; ++s) {
@@ -509,6 +526,8 @@ static int build_models(pripps_engine *e, int32_t ns, const pripps_debye_input *
     } else {
         partial_bytes = std::max<size_t>(1, items.size()) * kProfiles * Q_common * 8;
         o_partials = L.take(partial_bytes);
+        const size_t nchunks_total = stage_ptr.empty() ? 0 : (size_t)stage_ptr.back();
+        o_stage = L.take(std::max<size_t>(1, nchunks_total) * kProfiles * Q_common * 8);
     }
     const size_t total_bytes = L.off;
 
@@ -552,6 +571,7 @@ static int build_models(pripps_engine *e, int32_t ns, const pripps_debye_input *
         d.nq = p.nq;
         d.K = p.K;
         d.uniform = p.uniform ? 1 : 0;
+        d.small_h = p.small_h ? 1 : 0;
         d.n_classes = p.n_classes;
         d.n_strips = Q_common ? (p.nq + Q_common - 1) / Q_common : 0;
         for (int c = 0; c <= p.n_classes; ++c) d.cls_start[c] = p.cls_start[c];
@@ -567,6 +587,7 @@ static int build_models(pripps_engine *e, int32_t ns, const pripps_debye_input *
     if (!csr_items.empty()) std::memcpy(hbase + o_csr_items, csr_items.data(), csr_items.size() * 4);
     if (!strip_struct.empty()) std::memcpy(hbase + o_ss, strip_struct.data(), strip_struct.size() * 4);
     if (!strip_index.empty()) std::memcpy(hbase + o_si, strip_index.data(), strip_index.size() * 4);
+    if (!stage_ptr.empty()) std::memcpy(hbase + o_sp, stage_ptr.data(), stage_ptr.size() * 4);
 
     auto run = [&]() -> int {
         hipStream_t st = e->stream;
@@ -615,6 +636,9 @@ static int build_models(pripps_engine *e, int32_t ns, const pripps_debye_input *
                 f.partials = (const double *)(dbase + o_partials);
                 f.strip_struct = (const int32_t *)(dbase + o_ss);
                 f.strip_index = (const int32_t *)(dbase + o_si);
+                f.stage_ptr = (const int32_t *)(dbase + o_sp);
+                f.stage = (double *)(dbase + o_stage);
+                f.max_chunks = max_chunks;
                 f.Q = Q_common;
                 HIPCHK(launch_finalize(f, (int)strip_struct.size(), st));
                 e->last_fin = f;

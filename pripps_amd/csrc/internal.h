@@ -11,6 +11,7 @@ namespace pripps {
 constexpr int kMaxKinds = 16;
 constexpr int kMaxClasses = 2 * kMaxKinds;
 constexpr int kProfiles = 6;  // DebyeTerms order: AA EE HH AE AH EH
+constexpr int kFinChunk = 32; // item partials summed per block in the first reduction stage
 
 // One atom of the sorted SoA-in-AoS stream: centred coordinates and the
 // hydration weight, 16 bytes (FP32) or 32 bytes (FP64) per atom so a wave
@@ -33,6 +34,7 @@ struct DevStruct {
     double sum_w2;          // sum of squared hydration weights
     double q0, dq;          // uniform grid: q_m = q0 + m dq
     int32_t n, nq, K, uniform;
+    int32_t small_h;        // dq r_max / 4 pi <= 1/8 revolution for every pair
     int32_t n_classes;
     int32_t n_strips;       // ceil(nq / Q)
     int64_t item_begin;     // first item of this structure (items are grouped
@@ -71,6 +73,9 @@ struct FinalizeArgs {
     const double *partials;
     const int32_t *strip_struct;  // flattened (structure, strip) -> structure
     const int32_t *strip_index;   // flattened (structure, strip) -> strip
+    const int32_t *stage_ptr;     // per (structure, strip): first chunk slot in `stage`
+    double *stage;                // [chunks][6][Q] chunk totals
+    int max_chunks;               // largest chunk count of any strip
     int Q;
 };
 

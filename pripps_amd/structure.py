@@ -93,4 +93,4 @@ def write_xyz(path, names: Sequence[str], pos: np.ndarray, comment: str = "") ->
     with open(path, "w", encoding="utf-8") as fh:
         fh.write(f"{len(names)}\n{comment}\n")
         for n, (x, y, z) in zip(names, pos):
-            fh.write(f"{n} {x!r} {y!r} {z!r}\n")
+            fh.write(f"{n} {float(x)!r} {float(y)!r} {float(z)!r}\n")
