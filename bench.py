@@ -61,6 +61,7 @@ def main():
     ap.add_argument("--q-per-lane", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--span", type=int, default=0)
+    ap.add_argument("--variant", type=int, default=0)
     ap.add_argument("--cpu-q", type=int, default=8, help="q points of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -86,7 +87,7 @@ def main():
     terms = wl.pair_q_terms
     log(f"workload {wl.name}: N={wl.n_atoms} nq={len(wl.q)} terms={terms:.4e} "
         f"(setup {time.time() - t_setup:.1f}s)")
-    tune = _native.tuning(args.q_per_lane, args.block, args.span)
+    tune = _native.tuning(args.q_per_lane, args.block, args.span, args.variant)
 
     # FP64 references on the GPU (also timed: the plain path is the baseline of
     # the batched/compensated target)

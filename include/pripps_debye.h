@@ -84,9 +84,10 @@ typedef struct pripps_debye_model pripps_debye_model;
 
 /* Kernel tuning; zero fields mean "choose automatically". */
 typedef struct {
-    int32_t q_per_lane;   /* q points per lane strip: 64 or 128 */
+    int32_t q_per_lane;   /* q points per lane strip: 32, 64 or 128 (FP64: 32, 64) */
     int32_t block;        /* pairs summed in FP32 before the FP64 carry */
     int32_t span;         /* atoms per work item along j */
+    int32_t variant;      /* FP32 inner-loop variant for the autotuner (0 = default) */
 } pripps_tuning;
 
 /* Result of a fit (reference FitResult, src/fit.rs:238-247). */

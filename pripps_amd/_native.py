@@ -48,7 +48,7 @@ class DebyeInput(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("q_per_lane", ctypes.c_int32), ("block", ctypes.c_int32),
-                ("span", ctypes.c_int32)]
+                ("span", ctypes.c_int32), ("variant", ctypes.c_int32)]
 
 
 class FitResultC(ctypes.Structure):
@@ -164,5 +164,5 @@ class PreparedInput:
                             dptr(self.weights))
 
 
-def tuning(q_per_lane=0, block=0, span=0):
-    return Tuning(int(q_per_lane), int(block), int(span))
+def tuning(q_per_lane=0, block=0, span=0, variant=0):
+    return Tuning(int(q_per_lane), int(block), int(span), int(variant))

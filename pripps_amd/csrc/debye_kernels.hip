@@ -212,11 +212,106 @@ __device__ __forceinline__ Seed64 seed_f64(double r2, double q0, double hdq, dou
 // ---------------------------------------------------------------------------
 // the tiled pair kernel
 // ---------------------------------------------------------------------------
-template <typename T, int Q, bool UNIFORM, bool WET, bool MASK>
+// Two consecutive j atoms, fetched with one 32-byte scalar load.
+struct alignas(16) AtomPair32 {
+    Atom<float> a, b;
+};
+
+__device__ __forceinline__ AtomPair32 load_pair(const PRIPPS_CONST Atom<float> *ca, int j) {
+    const PRIPPS_CONST float *f = (const PRIPPS_CONST float *)(ca + j);
+    AtomPair32 p;
+    p.a.x = f[0];
+    p.a.y = f[1];
+    p.a.z = f[2];
+    p.a.w = f[3];
+    p.b.x = f[4];
+    p.b.y = f[5];
+    p.b.z = f[6];
+    p.b.w = f[7];
+    return p;
+}
+
+// FP32, uniform q grid: two j atoms per iteration (one paired scalar load).
+// INTERLEAVE runs their two recurrence chains step by step together; otherwise
+// one after the other (which measured faster on gfx950).  Dry
+// chunks cost 3 VALU ops per pair-q term, hydrated chunks 4.  The diagonal
+// test j > i is always applied; padding lanes carry i = INT_MAX and so never
+// contribute.  Odd chunks end with one single-atom step.
+template <int Q, bool WET, bool SMALL_H, bool INTERLEAVE>
+__device__ __forceinline__ void pair_block_f32x2(const Atom<float> *__restrict__ atoms, int jb,
+                                                 int je, const Atom<float> &ai, int iidx,
+                                                 float q0p, float hp, float (&acc0)[Q],
+                                                 float (&accw)[Q]) {
+    const PRIPPS_CONST Atom<float> *ca = as_const(atoms);
+    int j = jb;
+    AtomPair32 nxt{};
+    if (je - jb >= 2) nxt = load_pair(ca, jb);  // never reads past je - 1
+    for (; j + 1 < je; j += 2) {
+        const AtomPair32 cur = nxt;
+        nxt = load_pair(ca, min(j + 2, je - 2));  // prefetch the next two atoms
+        const Atom<float> &a = cur.a, &b = cur.b;
+        const float ax = ai.x - a.x, ay = ai.y - a.y, az = ai.z - a.z;
+        const float bx = ai.x - b.x, by = ai.y - b.y, bz = ai.z - b.z;
+        const float ta = j > iidx ? 1.0f : 0.0f;
+        const float tb = j + 1 > iidx ? 1.0f : 0.0f;
+        const Seed32 za = seed_f32(ax * ax + ay * ay + az * az, q0p, hp, ta, SMALL_H);
+        const Seed32 zb = seed_f32(bx * bx + by * by + bz * bz, q0p, hp, tb, SMALL_H);
+        float s1 = za.s, d1 = za.d, s2 = zb.s, d2 = zb.d;
+        const float l1 = za.lam, l2 = zb.lam;
+        if constexpr (INTERLEAVE) {
+#pragma unroll
+            for (int k = 0; k < Q; ++k) {
+                acc0[k] += s1 + s2;
+                if constexpr (WET) {
+                    accw[k] = __builtin_fmaf(a.w, s1, accw[k]);
+                    accw[k] = __builtin_fmaf(b.w, s2, accw[k]);
+                }
+                s1 += d1;
+                s2 += d2;
+                d1 = __builtin_fmaf(-l1, s1, d1);
+                d2 = __builtin_fmaf(-l2, s2, d2);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < Q; ++k) {
+                acc0[k] += s1;
+                if constexpr (WET) accw[k] = __builtin_fmaf(a.w, s1, accw[k]);
+                s1 += d1;
+                d1 = __builtin_fmaf(-l1, s1, d1);
+            }
+#pragma unroll
+            for (int k = 0; k < Q; ++k) {
+                acc0[k] += s2;
+                if constexpr (WET) accw[k] = __builtin_fmaf(b.w, s2, accw[k]);
+                s2 += d2;
+                d2 = __builtin_fmaf(-l2, s2, d2);
+            }
+        }
+    }
+    if (j < je) {
+        const Atom<float> a = load_atom(ca, j);
+        const float ax = ai.x - a.x, ay = ai.y - a.y, az = ai.z - a.z;
+        const float ta = j > iidx ? 1.0f : 0.0f;
+        const Seed32 za = seed_f32(ax * ax + ay * ay + az * az, q0p, hp, ta, SMALL_H);
+        float s1 = za.s, d1 = za.d;
+        const float l1 = za.lam;
+#pragma unroll
+        for (int k = 0; k < Q; ++k) {
+            acc0[k] += s1;
+            if constexpr (WET) accw[k] = __builtin_fmaf(a.w, s1, accw[k]);
+            s1 += d1;
+            d1 = __builtin_fmaf(-l1, s1, d1);
+        }
+    }
+}
+
+template <typename T, int Q, bool UNIFORM, bool WET, bool MASK, int SMALL_H>
 __device__ __forceinline__ void pair_block(const Atom<T> *__restrict__ atoms, int jb, int je,
                                            const Atom<T> &ai, int iidx, T lane_take,
-                                           const T (&qv)[Q], T q0p, T hp, bool small_h,
+                                           const T (&qv)[Q], T q0p, T hp, bool small_h_rt,
                                            T (&acc0)[Q], T (&accw)[Q]) {
+    // SMALL_H: 1 / 0 fixed at compile time, -1 decided per pair from small_h_rt
+    const bool small_h = SMALL_H < 0 ? small_h_rt : SMALL_H == 1;
     const PRIPPS_CONST Atom<T> *ca = as_const(atoms);
     Atom<T> next = load_atom(ca, jb);
     for (int j = jb; j < je; ++j) {
@@ -273,8 +368,12 @@ __device__ __forceinline__ void pair_block(const Atom<T> *__restrict__ atoms, in
     }
 }
 
-template <typename T, int Q, bool UNIFORM>
+// VARIANT (FP32 uniform grids only): 1 paired loads, chains one after the other;
+// 2 paired loads, chains interleaved; 3 one atom per iteration, half-step
+// sine path chosen per pair; 4 one atom per iteration, path fixed per chunk.
+template <typename T, int Q, bool UNIFORM, int VARIANT>
 __global__ void __launch_bounds__(256) debye_tiled_kernel(TiledArgs args) {
+    constexpr bool INTERLEAVE = VARIANT == 2;
     constexpr int M = Split<Q>::M;
     const int lane = __lane_id();
     const bool owner = split_owner<Q>(lane);
@@ -347,12 +446,32 @@ __global__ void __launch_bounds__(256) debye_tiled_kernel(TiledArgs args) {
 #pragma unroll
             for (int k = 0; k < Q; ++k) acc0[k] = accw[k] = T(0);
 
-            if (wet) {
-                if (mask) pair_block<T, Q, UNIFORM, true, true>(atoms, j, jend, ai, iidx, lane_take, qv, q0p, hp, small_h, acc0, accw);
-                else pair_block<T, Q, UNIFORM, true, false>(atoms, j, jend, ai, iidx, lane_take, qv, q0p, hp, small_h, acc0, accw);
+            if constexpr (UNIFORM && sizeof(T) == 4 && (VARIANT == 1 || VARIANT == 2)) {
+                if (wet) {
+                    if (small_h) pair_block_f32x2<Q, true, true, INTERLEAVE>(atoms, j, jend, ai, iidx, q0p, hp, acc0, accw);
+                    else pair_block_f32x2<Q, true, false, INTERLEAVE>(atoms, j, jend, ai, iidx, q0p, hp, acc0, accw);
+                } else {
+                    if (small_h) pair_block_f32x2<Q, false, true, INTERLEAVE>(atoms, j, jend, ai, iidx, q0p, hp, acc0, accw);
+                    else pair_block_f32x2<Q, false, false, INTERLEAVE>(atoms, j, jend, ai, iidx, q0p, hp, acc0, accw);
+                }
+            } else if constexpr (UNIFORM && sizeof(T) == 4 && VARIANT == 4) {
+#define PRIPPS_PB(W, M, S) pair_block<T, Q, UNIFORM, W, M, S>(atoms, j, jend, ai, iidx, lane_take, qv, q0p, hp, small_h, acc0, accw)
+                if (wet) {
+                    if (mask) { if (small_h) PRIPPS_PB(true, true, 1); else PRIPPS_PB(true, true, 0); }
+                    else { if (small_h) PRIPPS_PB(true, false, 1); else PRIPPS_PB(true, false, 0); }
+                } else {
+                    if (mask) { if (small_h) PRIPPS_PB(false, true, 1); else PRIPPS_PB(false, true, 0); }
+                    else { if (small_h) PRIPPS_PB(false, false, 1); else PRIPPS_PB(false, false, 0); }
+                }
             } else {
-                if (mask) pair_block<T, Q, UNIFORM, false, true>(atoms, j, jend, ai, iidx, lane_take, qv, q0p, hp, small_h, acc0, accw);
-                else pair_block<T, Q, UNIFORM, false, false>(atoms, j, jend, ai, iidx, lane_take, qv, q0p, hp, small_h, acc0, accw);
+                if (wet) {
+                    if (mask) PRIPPS_PB(true, true, -1);
+                    else PRIPPS_PB(true, false, -1);
+                } else {
+                    if (mask) PRIPPS_PB(false, true, -1);
+                    else PRIPPS_PB(false, false, -1);
+                }
+#undef PRIPPS_PB
             }
 
             // FP64 carry: wave sums of S = sum s, Si = sum w_i s, Sw = sum w_j s,
@@ -612,9 +731,9 @@ __global__ void combine_kernel(const double *__restrict__ prof, int nq, double v
 // ---------------------------------------------------------------------------
 // launch wrappers (host side of this translation unit)
 // ---------------------------------------------------------------------------
-template <typename T, int Q, bool U>
+template <typename T, int Q, bool U, int V = 3>
 static hipError_t launch_tiled_t(const TiledArgs &a, int blocks, hipStream_t st) {
-    hipLaunchKernelGGL((debye_tiled_kernel<T, Q, U>), dim3(blocks), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((debye_tiled_kernel<T, Q, U, V>), dim3(blocks), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
@@ -622,6 +741,10 @@ hipError_t launch_tiled(bool fp64, int Q, bool uniform, const TiledArgs &a, int 
                         hipStream_t st) {
     // Instantiated shapes: those whose accumulators fit the 256 VGPRs of a wave.
     if (!fp64) {
+        if (uniform && a.variant == 1 && Q == 64) return launch_tiled_t<float, 64, true, 1>(a, blocks, st);
+        if (uniform && a.variant == 2 && Q == 64) return launch_tiled_t<float, 64, true, 2>(a, blocks, st);
+        if (uniform && a.variant == 4 && Q == 64) return launch_tiled_t<float, 64, true, 4>(a, blocks, st);
+        if (uniform && a.variant == 4 && Q == 32) return launch_tiled_t<float, 32, true, 4>(a, blocks, st);
         if (Q == 32) return uniform ? launch_tiled_t<float, 32, true>(a, blocks, st)
                                     : launch_tiled_t<float, 32, false>(a, blocks, st);
         if (Q == 64) return uniform ? launch_tiled_t<float, 64, true>(a, blocks, st)
@@ -638,7 +761,7 @@ hipError_t launch_tiled(bool fp64, int Q, bool uniform, const TiledArgs &a, int 
 template <typename T, int Q, bool U>
 static hipError_t occupancy_t(int *blocks_per_cu) {
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        blocks_per_cu, reinterpret_cast<const void *>(&debye_tiled_kernel<T, Q, U>), 256, 0);
+        blocks_per_cu, reinterpret_cast<const void *>(&debye_tiled_kernel<T, Q, U, 3>), 256, 0);
 }
 
 hipError_t tiled_occupancy(bool fp64, int Q, bool uniform, int *blocks_per_cu) {

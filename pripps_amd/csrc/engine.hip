@@ -619,6 +619,7 @@ static int build_models(pripps_engine *e, int32_t ns, const pripps_debye_input *
                 a.counter = counter;
                 a.partials = (double *)(dbase + o_partials);
                 a.block = block;
+                a.variant = (tune && tune->variant > 0) ? tune->variant : 3;
                 int bpc = 1;
                 HIPCHK(tiled_occupancy(fp64, Q_common, uniform_all, &bpc));
                 bpc = std::max(1, bpc);

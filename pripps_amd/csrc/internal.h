@@ -64,6 +64,7 @@ struct TiledArgs {
     int *counter;      // work-queue head, zeroed before launch
     double *partials;  // [n_items][6][Q]
     int block;         // pairs summed in FP32 before the FP64 carry
+    int variant;       // FP32 uniform inner-loop variant (see debye_tiled_kernel)
 };
 
 struct FinalizeArgs {
