@@ -65,6 +65,13 @@ def main():
     ap.add_argument("--cpu-q", type=int, default=8, help="q points of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--workload", default="solvent10k",
+                    choices=("solvent10k", "vacuum1k", "helix60k", "batch512", "served"))
+    ap.add_argument("--count", type=int, default=512, help="batch512: structures")
+    ap.add_argument("--plain-sample", type=int, default=32,
+                    help="batch512: structures timed on the sequential plain FP64 path")
+    ap.add_argument("--rates", default="10,50,100,200,500,1000,2000")
+    ap.add_argument("--requests", type=int, default=100, help="served: requests per rate")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
@@ -75,6 +82,10 @@ def main():
     import pripps_amd
     from pripps_amd import _native
     from pripps_amd.synth import solvent10k
+
+    if args.workload != "solvent10k":
+        WORKLOADS[args.workload](args)
+        return
 
     t_setup = time.time()
     wl = solvent10k(n=args.n, nq=args.nq)
@@ -178,6 +189,237 @@ def main():
         "extra": extra,
     }
     print(json.dumps(out), flush=True)
+
+
+# ---------------------------------------------------------------------------
+# the other benchmark configurations (one JSON line each)
+# ---------------------------------------------------------------------------
+def _engine_and_ff():
+    import pripps_amd
+    eng = pripps_amd.Engine(0)
+    yaml_path = os.path.join(HERE, "tests", "golden", "reference_data", "foxs_formfactors.yaml")
+    return eng, pripps_amd.Pripps(yaml_path, engine=eng)
+
+
+def _oracle():
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import oracle as O
+    return O
+
+
+def _per_q_error(got, ref):
+    """Envelope error per q (max over the six profiles)."""
+    import numpy as np
+    err = np.zeros(got.shape[1])
+    for p, (x, y) in enumerate(PAIRS):
+        env = np.sqrt(np.abs(ref[x]) * np.abs(ref[y]))
+        env = np.where(env > 0, env, 1.0)
+        err = np.maximum(err, np.abs(got[p] - ref[p]) / env)
+    return err
+
+
+def _timed(model, steps, warmup):
+    if warmup:
+        model.recompute(warmup)
+    t0 = time.perf_counter()
+    pair_ms, _ = model.recompute(steps)
+    return (time.perf_counter() - t0) * 1e3 / max(steps, 1), pair_ms
+
+
+def bench_vacuum1k(args):
+    """1,000-atom globule in vacuum, nq = 500: FP64 modes against the oracle."""
+    import numpy as np
+    from pripps_amd.synth import vacuum1k
+    O = _oracle()
+    eng, ff = _engine_and_ff()
+    wl = vacuum1k()
+    inp = ff.prepare_workload(wl)
+    terms = wl.pair_q_terms
+    threads = min(16, os.cpu_count() or 1)
+    tc = time.perf_counter()
+    ref_order = O.build_model(wl.pos, inp.kind, inp.q, inp.atom_ff, mode=O.REFERENCE_ORDER,
+                              nthreads=threads)
+    tcpu = time.perf_counter() - tc
+    ref = O.build_model(wl.pos, inp.kind, inp.q, inp.atom_ff, mode=O.COMPENSATED, nthreads=threads)
+    out = {"workload": "vacuum1k: 1,000-atom globule R=17 A, vacuum, q uniform [0,0.5] nq=500",
+           "pair_q_terms": terms,
+           "cpu_reference_order": {"value": terms / tcpu, "unit": UNIT, "cores": threads,
+                                   "kind": "port", "seconds": tcpu,
+                                   "max_err_vs_compensated": envelope_error(ref_order, ref)}}
+    for prec in ("fp64_plain", "fp64", "fp32"):
+        m = eng.build(inp, prec)
+        prof = m.profiles()
+        rec = {"max_err_vs_oracle": envelope_error(prof, ref), "build_kernel_ms": m.stats().kernel_ms}
+        if prec != "fp64_plain":
+            ms, pair_ms = _timed(m, args.steps, args.warmup)
+            rec.update({"ms_per_step": ms, "pair_kernel_ms": pair_ms, "terms_per_s": terms / (ms * 1e-3)})
+        else:
+            rec["terms_per_s"] = terms / (rec["build_kernel_ms"] * 1e-3)
+        out[prec] = rec
+        log(f"vacuum1k {prec}: {rec}")
+    print(json.dumps(out), flush=True)
+
+
+def bench_helix60k(args):
+    """Six 10k-atom globules on a helix, log-spaced q (no recurrence)."""
+    import numpy as np
+    from pripps_amd.synth import helix60k
+    O = _oracle()
+    eng, ff = _engine_and_ff()
+    wl = helix60k()
+    inp = ff.prepare_workload(wl)
+    terms = wl.pair_q_terms
+    out = {"workload": "helix60k: 6 x 10,000-atom globules on a helix (pitch 60 A, 3/4 turn per "
+                       "globule, radius 45 A), solvent, q log-spaced [0.005,1.0] nq=1000",
+           "pair_q_terms": terms}
+    profs = {}
+    for prec in ("fp64", "fp32"):
+        m = eng.build(inp, prec)
+        profs[prec] = m.profiles()
+        ms, pair_ms = _timed(m, max(1, args.steps // 10), 0)
+        out[prec] = {"ms_per_step": ms, "pair_kernel_ms": pair_ms, "terms_per_s": terms / (ms * 1e-3),
+                     "build_kernel_ms": m.stats().kernel_ms}
+        log(f"helix60k {prec}: {out[prec]}")
+    perq = _per_q_error(profs["fp32"], profs["fp64"])
+    out["fp32_vs_fp64_per_q"] = {"max": float(perq.max()), "median": float(np.median(perq)),
+                                 "q_at_max": float(wl.q[int(perq.argmax())])}
+    sel = np.array([150, 500, 900])
+    threads = min(16, os.cpu_count() or 1)
+    tc = time.perf_counter()
+    ref = O.build_model(wl.pos, inp.kind, inp.q[sel], inp.atom_ff[:, sel], inp.excluded_ff[:, sel],
+                        inp.water_ff[sel], inp.weights, mode=O.COMPENSATED, nthreads=threads)
+    out["oracle_sampled_q"] = {"q": [float(x) for x in wl.q[sel]], "seconds": time.perf_counter() - tc,
+                               "fp64_max_err": envelope_error(profs["fp64"][:, sel], ref),
+                               "fp32_max_err": envelope_error(profs["fp32"][:, sel], ref)}
+    log(f"helix60k oracle: {out['oracle_sampled_q']}")
+    print(json.dumps(out), flush=True)
+
+
+def _make_batch(count, seed=4):
+    from concurrent.futures import ProcessPoolExecutor
+    from pripps_amd import synth
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    ns = np.exp(rng.uniform(math.log(500), math.log(20000), size=count)).astype(int)
+    nqs = rng.integers(100, 1001, size=count)
+    qmaxs = rng.uniform(0.3, 0.8, size=count)
+    jobs = [(int(ns[s]), synth.radius_for(int(ns[s])), seed * 100000 + s) for s in range(count)]
+    with ProcessPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        gl = list(ex.map(_globule_job, jobs))
+    return [synth.Workload(f"batch{s}", gl[s][0], gl[s][1], synth.linear_q(0.0, float(qmaxs[s]), int(nqs[s])),
+                           gl[s][2]) for s in range(count)]
+
+
+def _globule_job(job):
+    from pripps_amd.synth import globule
+    n, r, seed = job
+    return globule(n, r, seed, solvent=True, jitter=0.5)
+
+
+def bench_batch512(args):
+    """512 structures of uneven size and q grid in one persistent launch,
+    against the plain FP64 path launched once per structure."""
+    import numpy as np
+    eng, ff = _engine_and_ff()
+    t = time.perf_counter()
+    wls = _make_batch(args.count)
+    inps = [ff.prepare_workload(w) for w in wls]
+    terms = float(sum(w.pair_q_terms for w in wls))
+    log(f"batch: {len(wls)} structures, {terms:.4e} terms (setup {time.perf_counter() - t:.1f} s)")
+    models = eng.build_batch(inps, "fp32")
+    first_ms = models[0].stats().kernel_ms
+    ms, pair_ms = _timed(models[0], max(1, args.steps // 4), 1)
+    out = {"workload": f"batch{len(wls)}: globules N log-uniform [500,20000], jitter 0.5 A, own q grid "
+                       "nq U[100,1000] qmax U[0.3,0.8], solvent",
+           "structures": len(wls), "pair_q_terms": terms,
+           "fp32_batch": {"ms_per_step": ms, "pair_kernel_ms": pair_ms, "first_build_kernel_ms": first_ms,
+                          "terms_per_s": terms / (ms * 1e-3)}}
+    log(f"fp32 batch: {out['fp32_batch']}")
+    m64 = eng.build_batch(inps, "fp64")
+    ms64, pair64 = _timed(m64[0], 1, 0)
+    out["fp64_batch"] = {"ms_per_step": ms64, "pair_kernel_ms": pair64, "terms_per_s": terms / (ms64 * 1e-3)}
+    errs = [envelope_error(a.profiles(), b.profiles()) for a, b in zip(models, m64)]
+    out["fp32_vs_fp64_max_err"] = float(max(errs))
+    del m64
+    # sequential plain FP64 launches on an evenly spaced sample, extrapolated by terms
+    k = min(args.plain_sample, len(wls))
+    sel = np.unique(np.linspace(0, len(wls) - 1, k).round().astype(int))
+    plain_ms, plain_terms = 0.0, 0.0
+    for s in sel:
+        m = eng.build(inps[s], "fp64_plain")
+        plain_ms += m.stats().kernel_ms
+        plain_terms += wls[s].pair_q_terms
+    plain_rate = plain_terms / (plain_ms * 1e-3)
+    out["fp64_plain_sequential"] = {"sample": int(len(sel)), "sample_terms": plain_terms,
+                                    "sample_kernel_ms": plain_ms, "terms_per_s": plain_rate,
+                                    "extrapolated_ms": terms / plain_rate * 1e3}
+    out["speedup_vs_sequential_plain"] = out["fp32_batch"]["terms_per_s"] / plain_rate
+    log(f"plain sequential sample: {out['fp64_plain_sequential']}; speedup {out['speedup_vs_sequential_plain']:.1f}x")
+    print(json.dumps(out), flush=True)
+
+
+def _request_job(job):
+    from pripps_amd.synth import globule, radius_for
+    n, seed = job
+    return globule(n, radius_for(n), seed, solvent=True)
+
+
+def bench_served(args):
+    """Poisson request streams at several rates through the serving loop."""
+    import numpy as np
+    from concurrent.futures import ProcessPoolExecutor
+    from pripps_amd.server import Server
+    from pripps_amd.synth import Workload, linear_q, request_stream
+    eng, ff = _engine_and_ff()
+    rates = [float(r) for r in args.rates.split(",")]
+    total = args.requests * len(rates)
+    ns, nqs, knobs, seeds = request_stream(total)
+    t = time.perf_counter()
+    with ProcessPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        gl = list(ex.map(_request_job, [(int(ns[i]), int(seeds[i])) for i in range(total)]))
+    inps = [ff.prepare_workload(Workload(f"r{i}", gl[i][0], gl[i][1], linear_q(0.0, 0.5, int(nqs[i])), gl[i][2]))
+            for i in range(total)]
+    terms = np.array([ns[i] * (ns[i] - 1) / 2 * nqs[i] for i in range(total)], dtype=float)
+    log(f"served: {total} requests prepared in {time.perf_counter() - t:.1f} s, "
+        f"mean {terms.mean():.3e} terms/request")
+    rng = np.random.default_rng(11)
+    results = []
+    with Server(eng, "fp32") as srv:
+        # warm both lanes
+        for i in range(2):
+            srv.wait(srv.submit(inps[i]))
+        for ri, rate in enumerate(rates):
+            idx = list(range(ri * args.requests, (ri + 1) * args.requests))
+            arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(idx)))
+            t0 = time.perf_counter()
+            rids = []
+            for k, i in enumerate(idx):
+                delay = t0 + arrivals[k] - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                rids.append((i, srv.submit(inps[i]), time.perf_counter()))
+            lat = []
+            for i, rid, ts in rids:
+                m, build_lat = srv.wait(rid)
+                for vs, cd in knobs[i]:
+                    m.intensity(vs, cd)
+                lat.append((time.perf_counter() - ts) * 1e3)
+                m.close()
+            t1 = time.perf_counter()
+            lat = np.array(lat)
+            rec = {"offered_rps": rate, "requests": len(idx), "wall_s": t1 - t0,
+                   "completed_rps": len(idx) / (t1 - t0), "terms_per_s": float(terms[idx].sum() / (t1 - t0)),
+                   "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))}
+            results.append(rec)
+            log(f"served rate {rate}: {rec}")
+    print(json.dumps({"workload": "served: Poisson request streams, globules N log-U[300,30000], "
+                                  "nq U[200,800], q [0,0.5], solvent, +5 intensity() per request",
+                      "latency": "submit to model ready plus 5 intensity() calls",
+                      "rates": results}), flush=True)
+
+
+WORKLOADS = {"vacuum1k": bench_vacuum1k, "helix60k": bench_helix60k, "batch512": bench_batch512,
+             "served": bench_served}
 
 
 if __name__ == "__main__":

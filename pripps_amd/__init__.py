@@ -211,6 +211,14 @@ class Pripps:
     def names_to_ids(self, names: Sequence[str]) -> np.ndarray:
         return resolve_ids(names, self.formfactors.kinds())
 
+    def prepare_workload(self, wl, bulk_electron_density: float = 0.334) -> _native.PreparedInput:
+        """Host tables of a synthetic workload (``pripps_amd.synth.Workload``)."""
+        ids = self.names_to_ids(wl.names)
+        hydration = "weights" if wl.weights is not None else None
+        return self.prepare_input(wl.pos, ids, wl.q, excluded=wl.excluded, hydration=hydration,
+                                  hydration_weights=wl.weights,
+                                  bulk_electron_density=bulk_electron_density)
+
 
 __all__ = [
     "Pripps", "DebyeModel", "Intensity", "FitResult", "Excluded", "Hydration", "FitMethod",

@@ -169,14 +169,24 @@ __device__ __forceinline__ float sin_small_rev(float x) {
     return h + h * (h2 * p);
 }
 
-// FP32 seed: q0rev = q_s / 2pi, hrev = dq / 4pi (revolutions per Angstrom).
-// small_h: the host guarantees hrev * r <= 1/8 for every pair of the build.
-__device__ __forceinline__ Seed32 seed_f32(float r2, float q0rev, float hrev, float take,
-                                           bool small_h) {
+// 1/sqrt(r2) with one Newton step on the hardware estimate: the estimate's
+// relative error (and its bias) is squared away, leaving the rounding of the
+// last operations.  A bias in r acts as a q-scale error on every pair at once,
+// which the cancellation at deep minima of I(q) would amplify.
+__device__ __forceinline__ float rsqrt_refined(float r2) {
+    const float y = __builtin_amdgcn_rsqf(r2);
+    const float hx = 0.5f * r2;
+    return y * __builtin_fmaf(-hx * y, y, 1.5f);
+}
+
+// FP32 seed: q0rev = q_s / 2pi split as q0rev + q0lo (revolutions per A),
+// hrev = dq / 4pi.  small_h: the host guarantees hrev * r <= 1/8 for every pair.
+__device__ __forceinline__ Seed32 seed_f32(float r2, float q0rev, float q0lo, float hrev,
+                                           float take, bool small_h) {
     r2 = fmaxf(r2, 1e-12f);  // coincident atoms: the r -> 0 limit s_k -> q_k
-    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float rinv = rsqrt_refined(r2);
     const float r = r2 * rinv;
-    const float arev = q0rev * r;
+    const float arev = __builtin_fmaf(q0rev, r, q0lo * r);
     const float hr = hrev * r;
     const float sa = __builtin_amdgcn_sinf(__builtin_amdgcn_fractf(arev));
     const float cah = __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(arev + hr));
@@ -240,8 +250,8 @@ __device__ __forceinline__ AtomPair32 load_pair(const PRIPPS_CONST Atom<float> *
 template <int Q, bool WET, bool SMALL_H, bool INTERLEAVE>
 __device__ __forceinline__ void pair_block_f32x2(const Atom<float> *__restrict__ atoms, int jb,
                                                  int je, const Atom<float> &ai, int iidx,
-                                                 float q0p, float hp, float (&acc0)[Q],
-                                                 float (&accw)[Q]) {
+                                                 float q0p, float q0lo, float hp,
+                                                 float (&acc0)[Q], float (&accw)[Q]) {
     const PRIPPS_CONST Atom<float> *ca = as_const(atoms);
     int j = jb;
     AtomPair32 nxt{};
@@ -254,8 +264,8 @@ __device__ __forceinline__ void pair_block_f32x2(const Atom<float> *__restrict__
         const float bx = ai.x - b.x, by = ai.y - b.y, bz = ai.z - b.z;
         const float ta = j > iidx ? 1.0f : 0.0f;
         const float tb = j + 1 > iidx ? 1.0f : 0.0f;
-        const Seed32 za = seed_f32(ax * ax + ay * ay + az * az, q0p, hp, ta, SMALL_H);
-        const Seed32 zb = seed_f32(bx * bx + by * by + bz * bz, q0p, hp, tb, SMALL_H);
+        const Seed32 za = seed_f32(ax * ax + ay * ay + az * az, q0p, q0lo, hp, ta, SMALL_H);
+        const Seed32 zb = seed_f32(bx * bx + by * by + bz * bz, q0p, q0lo, hp, tb, SMALL_H);
         float s1 = za.s, d1 = za.d, s2 = zb.s, d2 = zb.d;
         const float l1 = za.lam, l2 = zb.lam;
         if constexpr (INTERLEAVE) {
@@ -292,7 +302,7 @@ __device__ __forceinline__ void pair_block_f32x2(const Atom<float> *__restrict__
         const Atom<float> a = load_atom(ca, j);
         const float ax = ai.x - a.x, ay = ai.y - a.y, az = ai.z - a.z;
         const float ta = j > iidx ? 1.0f : 0.0f;
-        const Seed32 za = seed_f32(ax * ax + ay * ay + az * az, q0p, hp, ta, SMALL_H);
+        const Seed32 za = seed_f32(ax * ax + ay * ay + az * az, q0p, q0lo, hp, ta, SMALL_H);
         float s1 = za.s, d1 = za.d;
         const float l1 = za.lam;
 #pragma unroll
@@ -308,8 +318,8 @@ __device__ __forceinline__ void pair_block_f32x2(const Atom<float> *__restrict__
 template <typename T, int Q, bool UNIFORM, bool WET, bool MASK, int SMALL_H>
 __device__ __forceinline__ void pair_block(const Atom<T> *__restrict__ atoms, int jb, int je,
                                            const Atom<T> &ai, int iidx, T lane_take,
-                                           const T (&qv)[Q], T q0p, T hp, bool small_h_rt,
-                                           T (&acc0)[Q], T (&accw)[Q]) {
+                                           const T (&qv)[Q], T q0p, T q0lo, T hp,
+                                           bool small_h_rt, T (&acc0)[Q], T (&accw)[Q]) {
     // SMALL_H: 1 / 0 fixed at compile time, -1 decided per pair from small_h_rt
     const bool small_h = SMALL_H < 0 ? small_h_rt : SMALL_H == 1;
     const PRIPPS_CONST Atom<T> *ca = as_const(atoms);
@@ -324,7 +334,7 @@ __device__ __forceinline__ void pair_block(const Atom<T> *__restrict__ atoms, in
         if constexpr (UNIFORM) {
             T s, d, lam;
             if constexpr (sizeof(T) == 4) {
-                Seed32 z = seed_f32(r2, q0p, hp, take, small_h);
+                Seed32 z = seed_f32(r2, q0p, q0lo, hp, take, small_h);
                 s = z.s;
                 d = z.d;
                 lam = z.lam;
@@ -344,7 +354,7 @@ __device__ __forceinline__ void pair_block(const Atom<T> *__restrict__ atoms, in
         } else {
             if constexpr (sizeof(T) == 4) {
                 const float rr2 = fmaxf(r2, 1e-12f);
-                const float rinv = __builtin_amdgcn_rsqf(rr2);
+                const float rinv = rsqrt_refined(rr2);
                 const float r = rr2 * rinv;
                 const float amp = take * rinv;
 #pragma unroll
@@ -400,11 +410,12 @@ __global__ void __launch_bounds__(256) debye_tiled_kernel(TiledArgs args) {
         const bool small_h = sd.small_h != 0;
         // strip constants
         T qv[Q];
-        T q0p = T(0), hp = T(0);
+        T q0p = T(0), q0lo = T(0), hp = T(0);
         if constexpr (UNIFORM) {
             const double qs = sd.q0 + double(qbase) * sd.dq;
             if constexpr (sizeof(T) == 4) {
                 q0p = float(qs * kInv2Pi);
+                q0lo = float(qs * kInv2Pi - double(q0p));
                 hp = float(sd.dq * (0.5 * kInv2Pi));
             } else {
                 q0p = qs;
@@ -448,14 +459,14 @@ __global__ void __launch_bounds__(256) debye_tiled_kernel(TiledArgs args) {
 
             if constexpr (UNIFORM && sizeof(T) == 4 && (VARIANT == 1 || VARIANT == 2)) {
                 if (wet) {
-                    if (small_h) pair_block_f32x2<Q, true, true, INTERLEAVE>(atoms, j, jend, ai, iidx, q0p, hp, acc0, accw);
-                    else pair_block_f32x2<Q, true, false, INTERLEAVE>(atoms, j, jend, ai, iidx, q0p, hp, acc0, accw);
+                    if (small_h) pair_block_f32x2<Q, true, true, INTERLEAVE>(atoms, j, jend, ai, iidx, q0p, q0lo, hp, acc0, accw);
+                    else pair_block_f32x2<Q, true, false, INTERLEAVE>(atoms, j, jend, ai, iidx, q0p, q0lo, hp, acc0, accw);
                 } else {
-                    if (small_h) pair_block_f32x2<Q, false, true, INTERLEAVE>(atoms, j, jend, ai, iidx, q0p, hp, acc0, accw);
-                    else pair_block_f32x2<Q, false, false, INTERLEAVE>(atoms, j, jend, ai, iidx, q0p, hp, acc0, accw);
+                    if (small_h) pair_block_f32x2<Q, false, true, INTERLEAVE>(atoms, j, jend, ai, iidx, q0p, q0lo, hp, acc0, accw);
+                    else pair_block_f32x2<Q, false, false, INTERLEAVE>(atoms, j, jend, ai, iidx, q0p, q0lo, hp, acc0, accw);
                 }
             } else if constexpr (UNIFORM && sizeof(T) == 4 && VARIANT == 4) {
-#define PRIPPS_PB(W, M, S) pair_block<T, Q, UNIFORM, W, M, S>(atoms, j, jend, ai, iidx, lane_take, qv, q0p, hp, small_h, acc0, accw)
+#define PRIPPS_PB(W, M, S) pair_block<T, Q, UNIFORM, W, M, S>(atoms, j, jend, ai, iidx, lane_take, qv, q0p, q0lo, hp, small_h, acc0, accw)
                 if (wet) {
                     if (mask) { if (small_h) PRIPPS_PB(true, true, 1); else PRIPPS_PB(true, true, 0); }
                     else { if (small_h) PRIPPS_PB(true, false, 1); else PRIPPS_PB(true, false, 0); }

@@ -61,6 +61,30 @@ extern "C" const char *pripps_version(void) { return "pripps_amd 0.1.0 (gfx950)"
 // ---------------------------------------------------------------------------
 // engine
 // ---------------------------------------------------------------------------
+namespace pripps {
+// One persistent launch of the pair kernel plus its reduction, replayable on
+// the inputs still resident in the engine arena.
+struct LaunchRecord {
+    bool fp64 = false, uniform = true;
+    int Q = 64, blocks = 0, n_strips = 0;
+    int64_t n_items = 0;
+    int *counter = nullptr;
+    TiledArgs targs{};
+    FinalizeArgs fargs{};
+};
+
+// fill counter, pair kernel (bracketed by ev_a/ev_b when given), reduction
+hipError_t run_launch(const LaunchRecord &r, hipStream_t st, hipEvent_t ev_a, hipEvent_t ev_b) {
+    hipError_t err = launch_fill_int(r.counter, 1, 0, st);
+    if (err == hipSuccess && ev_a) err = hipEventRecord(ev_a, st);
+    if (err == hipSuccess && r.n_items > 0)
+        err = launch_tiled(r.fp64, r.Q, r.uniform, r.targs, r.blocks, st);
+    if (err == hipSuccess && ev_b) err = hipEventRecord(ev_b, st);
+    if (err == hipSuccess && r.n_strips > 0) err = launch_finalize(r.fargs, r.n_strips, st);
+    return err;
+}
+}  // namespace pripps
+
 struct pripps_engine {
     std::atomic<int> refs{1};  // the creator's handle plus one per live model
     int device = 0;
@@ -72,14 +96,10 @@ struct pripps_engine {
     size_t arena_size = 0;
     void *staging = nullptr;  // pinned (hipHostMalloc)
     size_t staging_size = 0;
-    // launch state of the most recent build, valid while the arena holds it
+    // launches of the most recent build, valid while the arena holds them
     uint64_t build_serial = 0;
-    bool last_plain = false, last_fp64 = false, last_uniform = false;
-    int last_Q = 0, last_blocks = 0, last_strips = 0;
-    TiledArgs last_tiled{};
-    FinalizeArgs last_fin{};
-    int *last_counter = nullptr;
-    int64_t last_items = 0;
+    bool last_plain = false;
+    std::vector<LaunchRecord> last_launches;
 };
 
 struct pripps_debye_model {
@@ -376,6 +396,90 @@ static int choose_q(const pripps_tuning *tune, int precision, bool uniform, int 
     return nq <= 32 ? 32 : 64;
 }
 
+// j atoms per work item of one structure: about 8 items per wave slot of the
+// GPU when the structure runs alone, 256..8192, a multiple of 64.  It depends
+// on the structure only, so a structure is cut into the same items alone or in
+// any batch, and its profiles are bitwise the same either way.
+static int choose_span(const pripps_tuning *tune, const Prepared &p, int Q, int n_cu) {
+    if (tune && tune->span > 0) return std::max(64, tune->span);
+    const double strips = (double)((p.nq + Q - 1) / Q);
+    const double work = (double)p.tiles.size() * 0.5 * p.n * strips;
+    const double slots = std::max(1, n_cu) * 8.0;
+    const int span = (int)(work / (8.0 * slots));
+    return std::max(256, std::min(8192, (span + 63) / 64 * 64));
+}
+
+namespace {
+// Structures of one kernel shape (Q, uniform grid) share one persistent launch.
+struct LaunchGroup {
+    bool uniform = true;
+    int Q = 64;
+    std::vector<int> members;
+    std::vector<Item> items;
+    std::vector<int32_t> csr_ptr, csr_items, strip_struct, strip_index, stage_ptr;
+    int max_chunks = 0;
+    size_t o_items = 0, o_csr_ptr = 0, o_csr_items = 0, o_ss = 0, o_si = 0, o_sp = 0;
+    size_t o_counter = 0, o_partials = 0, o_stage = 0;
+};
+}  // namespace
+
+static void plan_group(LaunchGroup &g, const std::vector<Prepared> &P, const pripps_tuning *tune,
+                       int n_cu, std::vector<int> &span_of) {
+    for (int s : g.members) {
+        const Prepared &p = P[s];
+        const int nst = (p.nq + g.Q - 1) / g.Q;
+        const int span = choose_span(tune, p, g.Q, n_cu);
+        span_of[s] = span;
+        for (const int4 &t : p.tiles)
+            for (int32_t j0 = t.x; j0 < p.n; j0 += span) {
+                const int32_t j1 = std::min<int32_t>(p.n, j0 + span);
+                for (int st = 0; st < nst; ++st) g.items.push_back(Item{s, t.x, t.y, t.z, j0, j1, st, 0});
+            }
+    }
+    // largest first, by a stable counting sort on the span length (a
+    // structure's items keep one relative order in any batch)
+    {
+        int maxlen = 0;
+        for (const Item &it : g.items) maxlen = std::max(maxlen, it.j1 - it.j0);
+        std::vector<int64_t> pos(maxlen + 2, 0);
+        for (const Item &it : g.items) pos[maxlen - (it.j1 - it.j0) + 1]++;
+        for (int x = 0; x <= maxlen; ++x) pos[x + 1] += pos[x];
+        std::vector<Item> sorted(g.items.size());
+        for (const Item &it : g.items) sorted[pos[maxlen - (it.j1 - it.j0)]++] = it;
+        g.items.swap(sorted);
+    }
+    // CSR of items per (member, strip), in queue order
+    const int nm = (int)g.members.size();
+    std::vector<int64_t> base(nm + 1, 0);
+    std::vector<int> local(P.size(), -1);
+    for (int k = 0; k < nm; ++k) {
+        local[g.members[k]] = k;
+        base[k + 1] = base[k] + (P[g.members[k]].nq + g.Q - 1) / g.Q;
+    }
+    const int64_t n_strips = base[nm];
+    std::vector<int32_t> cnt(n_strips + 1, 0);
+    for (const Item &it : g.items) cnt[base[local[it.s]] + it.strip + 1]++;
+    for (int64_t x = 0; x < n_strips; ++x) cnt[x + 1] += cnt[x];
+    g.csr_ptr.assign(cnt.begin(), cnt.end());
+    g.csr_items.assign(g.items.size(), 0);
+    std::vector<int32_t> fillp(cnt.begin(), cnt.end() - 1);
+    for (size_t w = 0; w < g.items.size(); ++w)
+        g.csr_items[fillp[base[local[g.items[w].s]] + g.items[w].strip]++] = (int32_t)w;
+    g.stage_ptr.assign(n_strips + 1, 0);
+    for (int64_t x = 0; x < n_strips; ++x) {
+        const int nch = (g.csr_ptr[x + 1] - g.csr_ptr[x] + kFinChunk - 1) / kFinChunk;
+        g.stage_ptr[x + 1] = g.stage_ptr[x] + nch;
+        g.max_chunks = std::max(g.max_chunks, nch);
+    }
+    g.strip_struct.resize(n_strips);
+    g.strip_index.resize(n_strips);
+    for (int k = 0; k < nm; ++k)
+        for (int64_t st = 0; st < base[k + 1] - base[k]; ++st) {
+            g.strip_struct[base[k] + st] = g.members[k];
+            g.strip_index[base[k] + st] = (int32_t)st;
+        }
+}
+
 static int build_models(pripps_engine *e, int32_t ns, const pripps_debye_input *inputs,
                         int32_t precision, const pripps_tuning *tune, pripps_debye_model **out) {
     const auto t_start = std::chrono::steady_clock::now();
@@ -398,6 +502,9 @@ static int build_models(pripps_engine *e, int32_t ns, const pripps_debye_input *
     }
     const bool fp64 = precision != PRIPPS_FP32_COMPENSATED;
     const bool plain = precision == PRIPPS_FP64_PLAIN;
+    int block = tune && tune->block > 0 ? tune->block : 64;
+    block = std::max(8, std::min(block, 4096));
+    const int variant = (tune && tune->variant > 0) ? tune->variant : 3;
 
     std::lock_guard<std::mutex> lock(e->mu);
     HIPCHK(hipSetDevice(e->device));
@@ -426,70 +533,25 @@ static int build_models(pripps_engine *e, int32_t ns, const pripps_debye_input *
         }
     }
 
-    // ---- plan ----
-    int Q_common = 0;
-    bool uniform_all = true;
-    for (int s = 0; s < ns; ++s) uniform_all = uniform_all && P[s].uniform;
-    std::vector<Item> items;
-    std::vector<int32_t> strip_struct, strip_index, csr_ptr, csr_items, stage_ptr;
-    int max_chunks = 0;
-    int block = tune && tune->block > 0 ? tune->block : 64;
-    int span = tune && tune->span > 0 ? tune->span : 0;
-    block = std::max(8, std::min(block, 4096));
+    // ---- plan: one launch group per kernel shape ----
+    std::vector<LaunchGroup> groups;
+    std::vector<int> q_of(ns, 0), span_of(ns, 0);
     if (!plain) {
-        int maxnq = 0;
-        for (auto &p : P) maxnq = std::max(maxnq, p.nq);
-        Q_common = choose_q(tune, precision, uniform_all, maxnq);
-        // heuristic span: about 16 items per wave slot
-        if (span == 0) {
-            double pairs_tiles = 0;
-            for (auto &p : P)
-                pairs_tiles += (double)p.tiles.size() * 0.5 * p.n * ((p.nq + Q_common - 1) / Q_common);
-            const double slots = std::max(1, e->n_cu) * 8.0;
-            span = (int)(pairs_tiles / (16.0 * slots));
-            span = std::max(256, std::min(8192, (span + 63) / 64 * 64));
-        }
-        span = std::max(64, span);
         for (int s = 0; s < ns; ++s) {
-            const Prepared &p = P[s];
-            const int nst = (p.nq + Q_common - 1) / Q_common;
-            for (const int4 &t : p.tiles) {
-                for (int32_t j0 = t.x; j0 < p.n; j0 += span) {
-                    const int32_t j1 = std::min<int32_t>(p.n, j0 + span);
-                    for (int st = 0; st < nst; ++st)
-                        items.push_back(Item{s, t.x, t.y, t.z, j0, j1, st, 0});
-                }
+            const int Q = choose_q(tune, precision, P[s].uniform, P[s].nq);
+            q_of[s] = Q;
+            LaunchGroup *g = nullptr;
+            for (auto &x : groups)
+                if (x.Q == Q && x.uniform == P[s].uniform) g = &x;
+            if (!g) {
+                groups.emplace_back();
+                g = &groups.back();
+                g->Q = Q;
+                g->uniform = P[s].uniform;
             }
+            g->members.push_back(s);
         }
-        // largest first (stable), so the dynamic queue ends on small items
-        std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) {
-            return (a.j1 - a.j0) > (b.j1 - b.j0);
-        });
-        // CSR of items per (structure, strip)
-        std::vector<int64_t> base(ns + 1, 0);
-        for (int s = 0; s < ns; ++s) base[s + 1] = base[s] + (P[s].nq + Q_common - 1) / Q_common;
-        const int64_t n_strips_total = base[ns];
-        std::vector<int32_t> cnt(n_strips_total + 1, 0);
-        for (const Item &it : items) cnt[base[it.s] + it.strip + 1]++;
-        for (int64_t g = 0; g < n_strips_total; ++g) cnt[g + 1] += cnt[g];
-        csr_ptr.assign(cnt.begin(), cnt.end());
-        csr_items.assign(items.size(), 0);
-        std::vector<int32_t> fillp(cnt.begin(), cnt.end() - 1);
-        for (size_t w = 0; w < items.size(); ++w)
-            csr_items[fillp[base[items[w].s] + items[w].strip]++] = (int32_t)w;
-        stage_ptr.assign(n_strips_total + 1, 0);
-        for (int64_t g = 0; g < n_strips_total; ++g) {
-            const int nch = (csr_ptr[g + 1] - csr_ptr[g] + kFinChunk - 1) / kFinChunk;
-            stage_ptr[g + 1] = stage_ptr[g] + nch;
-            max_chunks = std::max(max_chunks, nch);
-        }
-        strip_struct.resize(n_strips_total);
-        strip_index.resize(n_strips_total);
-        for (int s = 0; s < ns; ++s)
-            for (int64_t st = 0; st < base[s + 1] - base[s]; ++st) {
-                strip_struct[base[s] + st] = s;
-                strip_index[base[s] + st] = (int32_t)st;
-            }
+        for (auto &g : groups) plan_group(g, P, tune, e->n_cu, span_of);
     }
 
     // ---- layout of the arena ----
@@ -504,30 +566,27 @@ static int build_models(pripps_engine *e, int32_t ns, const pripps_debye_input *
         o_ffw[s] = L.take(P[s].ffw.size() * 8 + 8);
         o_kn[s] = L.take(P[s].K * 8);
         o_kw[s] = L.take(P[s].K * 8);
-        if (plain) {
-            o_tiles[s] = L.take(P[s].tiles.size() * sizeof(int4) + 16);
-        }
+        if (plain) o_tiles[s] = L.take(P[s].tiles.size() * sizeof(int4) + 16);
     }
     const size_t o_structs = L.take(ns * sizeof(DevStruct));
-    const size_t o_items = L.take(items.size() * sizeof(Item) + 16);
-    const size_t o_csr_ptr = L.take(csr_ptr.size() * 4 + 4);
-    const size_t o_csr_items = L.take(csr_items.size() * 4 + 4);
-    const size_t o_ss = L.take(strip_struct.size() * 4 + 4);
-    const size_t o_si = L.take(strip_index.size() * 4 + 4);
-    const size_t o_sp = L.take(stage_ptr.size() * 4 + 4);
-    const size_t o_counter = L.take(256);
+    for (auto &g : groups) {
+        g.o_items = L.take(g.items.size() * sizeof(Item) + 16);
+        g.o_csr_ptr = L.take(g.csr_ptr.size() * 4 + 4);
+        g.o_csr_items = L.take(g.csr_items.size() * 4 + 4);
+        g.o_ss = L.take(g.strip_struct.size() * 4 + 4);
+        g.o_si = L.take(g.strip_index.size() * 4 + 4);
+        g.o_sp = L.take(g.stage_ptr.size() * 4 + 4);
+        g.o_counter = L.take(256);
+    }
     const size_t upload_bytes = L.off;
-    size_t o_partials = 0, o_stage = 0;
-    size_t partial_bytes = 0;
     if (plain) {
-        for (int s = 0; s < ns; ++s) {
+        for (int s = 0; s < ns; ++s)
             o_pp[s] = L.take(std::max<size_t>(1, P[s].tiles.size() * (size_t)P[s].nq * kProfiles) * 8);
-        }
-    } else {
-        partial_bytes = std::max<size_t>(1, items.size()) * kProfiles * Q_common * 8;
-        o_partials = L.take(partial_bytes);
-        const size_t nchunks_total = stage_ptr.empty() ? 0 : (size_t)stage_ptr.back();
-        o_stage = L.take(std::max<size_t>(1, nchunks_total) * kProfiles * Q_common * 8);
+    }
+    for (auto &g : groups) {
+        g.o_partials = L.take(std::max<size_t>(1, g.items.size()) * kProfiles * g.Q * 8);
+        const size_t nch = g.stage_ptr.empty() ? 0 : (size_t)g.stage_ptr.back();
+        g.o_stage = L.take(std::max<size_t>(1, nch) * kProfiles * g.Q * 8);
     }
     const size_t total_bytes = L.off;
 
@@ -573,7 +632,7 @@ static int build_models(pripps_engine *e, int32_t ns, const pripps_debye_input *
         d.uniform = p.uniform ? 1 : 0;
         d.small_h = p.small_h ? 1 : 0;
         d.n_classes = p.n_classes;
-        d.n_strips = Q_common ? (p.nq + Q_common - 1) / Q_common : 0;
+        d.n_strips = q_of[s] ? (p.nq + q_of[s] - 1) / q_of[s] : 0;
         for (int c = 0; c <= p.n_classes; ++c) d.cls_start[c] = p.cls_start[c];
         for (int c = p.n_classes + 1; c <= kMaxClasses; ++c) d.cls_start[c] = p.n;
         for (int c = 0; c < p.n_classes; ++c) {
@@ -582,12 +641,53 @@ static int build_models(pripps_engine *e, int32_t ns, const pripps_debye_input *
         }
     }
     std::memcpy(hbase + o_structs, ds.data(), ns * sizeof(DevStruct));
-    if (!items.empty()) std::memcpy(hbase + o_items, items.data(), items.size() * sizeof(Item));
-    if (!csr_ptr.empty()) std::memcpy(hbase + o_csr_ptr, csr_ptr.data(), csr_ptr.size() * 4);
-    if (!csr_items.empty()) std::memcpy(hbase + o_csr_items, csr_items.data(), csr_items.size() * 4);
-    if (!strip_struct.empty()) std::memcpy(hbase + o_ss, strip_struct.data(), strip_struct.size() * 4);
-    if (!strip_index.empty()) std::memcpy(hbase + o_si, strip_index.data(), strip_index.size() * 4);
-    if (!stage_ptr.empty()) std::memcpy(hbase + o_sp, stage_ptr.data(), stage_ptr.size() * 4);
+    auto put = [&](size_t off, const void *src, size_t bytes) {
+        if (bytes) std::memcpy(hbase + off, src, bytes);
+    };
+    for (auto &g : groups) {
+        put(g.o_items, g.items.data(), g.items.size() * sizeof(Item));
+        put(g.o_csr_ptr, g.csr_ptr.data(), g.csr_ptr.size() * 4);
+        put(g.o_csr_items, g.csr_items.data(), g.csr_items.size() * 4);
+        put(g.o_ss, g.strip_struct.data(), g.strip_struct.size() * 4);
+        put(g.o_si, g.strip_index.data(), g.strip_index.size() * 4);
+        put(g.o_sp, g.stage_ptr.data(), g.stage_ptr.size() * 4);
+    }
+
+    std::vector<LaunchRecord> launches;
+    for (auto &g : groups) {
+        LaunchRecord r;
+        r.fp64 = fp64;
+        r.Q = g.Q;
+        r.uniform = g.uniform;
+        r.counter = (int *)(dbase + g.o_counter);
+        r.n_items = (int64_t)g.items.size();
+        r.n_strips = (int)g.strip_struct.size();
+        TiledArgs &a = r.targs;
+        a.structs = (const DevStruct *)(dbase + o_structs);
+        a.items = (const Item *)(dbase + g.o_items);
+        a.n_items = r.n_items;
+        a.counter = r.counter;
+        a.partials = (double *)(dbase + g.o_partials);
+        a.block = block;
+        a.variant = variant;
+        int bpc = 1;
+        HIPCHK(tiled_occupancy(fp64, g.Q, g.uniform, &bpc));
+        bpc = std::max(1, bpc);
+        const int64_t want = (r.n_items + 3) / 4;
+        r.blocks = (int)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)e->n_cu * bpc));
+        FinalizeArgs &f = r.fargs;
+        f.structs = a.structs;
+        f.csr_ptr = (const int32_t *)(dbase + g.o_csr_ptr);
+        f.csr_items = (const int32_t *)(dbase + g.o_csr_items);
+        f.partials = a.partials;
+        f.strip_struct = (const int32_t *)(dbase + g.o_ss);
+        f.strip_index = (const int32_t *)(dbase + g.o_si);
+        f.stage_ptr = (const int32_t *)(dbase + g.o_sp);
+        f.stage = (double *)(dbase + g.o_stage);
+        f.max_chunks = g.max_chunks;
+        f.Q = g.Q;
+        launches.push_back(r);
+    }
 
     auto run = [&]() -> int {
         hipStream_t st = e->stream;
@@ -609,42 +709,7 @@ static int build_models(pripps_engine *e, int32_t ns, const pripps_debye_input *
                 HIPCHK(launch_plain(a, P[s].nq, st));
             }
         } else {
-            int *counter = (int *)(dbase + o_counter);
-            HIPCHK(launch_fill_int(counter, 1, 0, st));
-            if (!items.empty()) {
-                TiledArgs a;
-                a.structs = (const DevStruct *)(dbase + o_structs);
-                a.items = (const Item *)(dbase + o_items);
-                a.n_items = (int64_t)items.size();
-                a.counter = counter;
-                a.partials = (double *)(dbase + o_partials);
-                a.block = block;
-                a.variant = (tune && tune->variant > 0) ? tune->variant : 3;
-                int bpc = 1;
-                HIPCHK(tiled_occupancy(fp64, Q_common, uniform_all, &bpc));
-                bpc = std::max(1, bpc);
-                const int64_t want = ((int64_t)items.size() + 3) / 4;
-                const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)e->n_cu * bpc));
-                HIPCHK(launch_tiled(fp64, Q_common, uniform_all, a, blocks, st));
-                e->last_tiled = a;
-                e->last_blocks = blocks;
-            }
-            if (!strip_struct.empty()) {
-                FinalizeArgs f;
-                f.structs = (const DevStruct *)(dbase + o_structs);
-                f.csr_ptr = (const int32_t *)(dbase + o_csr_ptr);
-                f.csr_items = (const int32_t *)(dbase + o_csr_items);
-                f.partials = (const double *)(dbase + o_partials);
-                f.strip_struct = (const int32_t *)(dbase + o_ss);
-                f.strip_index = (const int32_t *)(dbase + o_si);
-                f.stage_ptr = (const int32_t *)(dbase + o_sp);
-                f.stage = (double *)(dbase + o_stage);
-                f.max_chunks = max_chunks;
-                f.Q = Q_common;
-                HIPCHK(launch_finalize(f, (int)strip_struct.size(), st));
-                e->last_fin = f;
-            }
-            e->last_counter = counter;
+            for (auto &r : launches) HIPCHK(run_launch(r, st, nullptr, nullptr));
         }
         HIPCHK(hipEventRecord(e->ev1, st));
         HIPCHK(hipStreamSynchronize(st));
@@ -652,15 +717,11 @@ static int build_models(pripps_engine *e, int32_t ns, const pripps_debye_input *
     };
     e->build_serial++;
     e->last_plain = plain;
-    e->last_fp64 = fp64;
-    e->last_uniform = uniform_all;
-    e->last_Q = Q_common;
-    e->last_items = (int64_t)items.size();
-    e->last_strips = (int)strip_struct.size();
-    e->last_blocks = 0;
+    e->last_launches = launches;
     rc = run();
     if (rc) {
         e->build_serial++;  // nothing valid to replay
+        e->last_launches.clear();
         cleanup();
         return rc;
     }
@@ -668,6 +729,8 @@ static int build_models(pripps_engine *e, int32_t ns, const pripps_debye_input *
     (void)hipEventElapsedTime(&ms, e->ev0, e->ev1);
     const double total_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    int64_t n_items = 0;
+    for (auto &g : groups) n_items += (int64_t)g.items.size();
     for (int s = 0; s < ns; ++s) {
         e->refs.fetch_add(1);
         models[s]->build_serial = e->build_serial;
@@ -675,10 +738,10 @@ static int build_models(pripps_engine *e, int32_t ns, const pripps_debye_input *
         S.kernel_ms = ms;
         S.total_ms = total_ms;
         S.pair_q_terms = P[s].pair_q_terms;
-        S.q_per_lane = Q_common;
+        S.q_per_lane = q_of[s];
         S.block = block;
-        S.span = span;
-        S.work_items = (int64_t)items.size();
+        S.span = span_of[s];
+        S.work_items = n_items;
         out[s] = models[s];
     }
     return PRIPPS_OK;
@@ -810,23 +873,18 @@ extern "C" int pripps_debye_model_recompute(pripps_debye_model *m, int32_t repea
                     "recompute needs a tiled model from the engine's most recent build");
     HIPCHK(hipSetDevice(e->device));
     hipStream_t st = e->stream;
-    std::vector<hipEvent_t> ev(2 * (size_t)repeats + 2, nullptr);
+    const size_t nl = e->last_launches.size();
+    std::vector<hipEvent_t> ev(2 * (size_t)repeats * nl + 2, nullptr);
     for (auto &x : ev) HIPCHK(hipEventCreate(&x));
     auto destroy = [&]() {
         for (auto &x : ev)
             if (x) (void)hipEventDestroy(x);
     };
     hipError_t err = hipEventRecord(ev[0], st);
-    for (int r = 0; r < repeats && err == hipSuccess; ++r) {
-        err = launch_fill_int(e->last_counter, 1, 0, st);
-        if (err == hipSuccess) err = hipEventRecord(ev[2 + 2 * r], st);
-        if (err == hipSuccess && e->last_items > 0)
-            err = launch_tiled(e->last_fp64, e->last_Q, e->last_uniform, e->last_tiled,
-                               e->last_blocks, st);
-        if (err == hipSuccess) err = hipEventRecord(ev[3 + 2 * r], st);
-        if (err == hipSuccess && e->last_strips > 0)
-            err = launch_finalize(e->last_fin, e->last_strips, st);
-    }
+    size_t k = 2;
+    for (int r = 0; r < repeats && err == hipSuccess; ++r)
+        for (size_t l = 0; l < nl && err == hipSuccess; ++l, k += 2)
+            err = run_launch(e->last_launches[l], st, ev[k], ev[k + 1]);
     if (err == hipSuccess) err = hipEventRecord(ev[1], st);
     if (err == hipSuccess) err = hipStreamSynchronize(st);
     if (err != hipSuccess) {
@@ -834,9 +892,9 @@ extern "C" int pripps_debye_model_recompute(pripps_debye_model *m, int32_t repea
         return hip_fail(err, "pripps_debye_model_recompute");
     }
     double sum = 0;
-    for (int r = 0; r < repeats; ++r) {
+    for (size_t x = 2; x < k; x += 2) {
         float ms = 0;
-        (void)hipEventElapsedTime(&ms, ev[2 + 2 * r], ev[3 + 2 * r]);
+        (void)hipEventElapsedTime(&ms, ev[x], ev[x + 1]);
         sum += ms;
     }
     float tot = 0;

@@ -1,0 +1,125 @@
+"""Parity at the benchmark configurations' full sizes (GPU).
+
+Sizes are the configurations' own; the oracle is evaluated on a few sampled
+q where the full grid would take minutes, and the remaining q are checked
+through the FP32-vs-FP64 agreement of the GPU modes.  Tolerances: FP32
+compensated 1e-5, FP64 1e-12 (envelope error, see test_gpu_parity.py).
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+from conftest import REFDATA
+from test_gpu_parity import envelope_error
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def foxs_engine(engine):
+    import pripps_amd
+    return pripps_amd.Pripps(os.path.join(REFDATA, "foxs_formfactors.yaml"), engine=engine)
+
+
+def oracle_at(oracle, inp, sel):
+    return oracle.build_model(np.column_stack([inp.x, inp.y, inp.z]), inp.kind, inp.q[sel],
+                              inp.atom_ff[:, sel],
+                              None if inp.excluded_ff is None else inp.excluded_ff[:, sel],
+                              None if inp.water_ff is None else inp.water_ff[sel], inp.weights,
+                              mode=oracle.COMPENSATED, nthreads=min(16, os.cpu_count() or 1))
+
+
+def test_vacuum1k_fp64(engine, oracle, foxs_engine):
+    from pripps_amd.synth import vacuum1k
+    wl = vacuum1k()
+    inp = foxs_engine.prepare_workload(wl)
+    assert inp.excluded_ff is None and inp.water_ff is None
+    ref = oracle_at(oracle, inp, np.arange(len(wl.q)))
+    for prec, tol in (("fp64", 1e-12), ("fp64_plain", 1e-12), ("fp32", 1e-5)):
+        print(f"vacuum1k {prec}", flush=True)
+        got = engine.build(inp, prec).profiles()
+        err = envelope_error(got, ref)
+        assert err.max() <= tol, (prec, err.max())
+
+
+def test_solvent10k(engine, oracle, foxs_engine):
+    from pripps_amd.synth import solvent10k
+    wl = solvent10k()
+    inp = foxs_engine.prepare_workload(wl)
+    sel = np.linspace(1, len(wl.q) - 1, 6).round().astype(int)
+    ref = oracle_at(oracle, inp, sel)
+    p32 = engine.build(inp, "fp32").profiles()
+    p64 = engine.build(inp, "fp64").profiles()
+    assert envelope_error(p32[:, sel], ref).max() <= 1e-5
+    assert envelope_error(p64[:, sel], ref).max() <= 1e-12
+    assert envelope_error(p32, p64).max() <= 1e-5
+    m = engine.build(inp, "fp32")
+    total = m.intensity(1.0, 0.03)[0]
+    assert np.array_equal(total, oracle.combine(m.profiles(), 1.0, 0.03))
+
+
+def test_helix60k_log_q(engine, oracle, foxs_engine):
+    from pripps_amd.synth import helix60k
+    wl = helix60k()
+    inp = foxs_engine.prepare_workload(wl)
+    sel = np.array([150, 900])
+    print("helix60k oracle", flush=True)
+    ref = oracle_at(oracle, inp, sel)
+    print("helix60k fp64", flush=True)
+    p64 = engine.build(inp, "fp64").profiles()
+    print("helix60k fp32", flush=True)
+    p32 = engine.build(inp, "fp32").profiles()
+    assert envelope_error(p64[:, sel], ref).max() <= 1e-12
+    assert envelope_error(p32[:, sel], ref).max() <= 1e-5
+    assert envelope_error(p32, p64).max() <= 1e-5
+
+
+def test_batch_subset_alone_equals_in_mix(engine, oracle, foxs_engine):
+    from pripps_amd.synth import batch
+    wls = batch(count=24)
+    inps = [foxs_engine.prepare_workload(w) for w in wls]
+    b32 = engine.build_batch(inps, "fp32")
+    b64 = engine.build_batch(inps, "fp64")
+    for k in (0, 5, 11, 23):
+        alone = engine.build(inps[k], "fp32").profiles()
+        assert np.array_equal(alone, b32[k].profiles())
+    for m32, m64 in zip(b32, b64):
+        assert envelope_error(m32.profiles(), m64.profiles()).max() <= 1e-5
+    k = int(np.argmax([w.n_atoms for w in wls]))
+    sel = np.array([1, len(wls[k].q) // 2, len(wls[k].q) - 1])
+    ref = oracle_at(oracle, inps[k], sel)
+    assert envelope_error(b64[k].profiles()[:, sel], ref).max() <= 1e-12
+
+
+def test_server_alone_equals_in_mix_and_admission(engine, foxs_engine):
+    from pripps_amd.errors import OutOfDeviceMemoryError
+    from pripps_amd.server import Server
+    from pripps_amd.synth import Workload, globule, linear_q, radius_for
+    rng = np.random.default_rng(8)
+    inps = []
+    for s in range(12):
+        n = int(rng.integers(300, 4000))
+        names, pos, w = globule(n, radius_for(n), 1000 + s)
+        wl = Workload(f"r{s}", names, pos, linear_q(0.0, 0.5, int(rng.integers(200, 800))), w)
+        inps.append(foxs_engine.prepare_workload(wl))
+    with Server(engine, "fp32") as srv:
+        ids = [srv.submit(i) for i in inps]
+        res = [srv.wait(r) for r in ids]
+    for inp, (m, lat) in zip(inps, res):
+        assert lat > 0
+        assert np.array_equal(m.profiles(), engine.build(inp, "fp32").profiles())
+        total = m.intensity(1.05, 0.5)[0]
+        assert np.all(np.isfinite(total))
+    # admission: a budget smaller than one request fails that request only
+    with Server(engine, "fp32", memory_budget=2_000_000) as srv:
+        small = srv.submit(inps[0])
+        m, _ = srv.wait(small)
+        assert m.nq == len(inps[0].q)
+        big = foxs_engine.prepare_workload(
+            Workload("big", *globule(20000, radius_for(20000), 77), linear_q(0, 0.5, 800)))
+        rid = srv.submit(big)
+        with pytest.raises(OutOfDeviceMemoryError):
+            srv.wait(rid)
