@@ -153,30 +153,77 @@ struct Seed32 {
     float s, d, lam;
 };
 
-// sin(2 pi x) for |x| <= 1/8 revolution as an odd polynomial in h = 2 pi x,
-// accurate to a few ulps *relative*: the recurrence constants
-// lambda = 4 sin^2(h) and delta_1 = 2 cos(a + h) sin(h) / r need sin(h) with
-// full relative accuracy, which the hardware sine (absolute accuracy) does not
-// give for the tiny h = dq r / 2 of fine q grids.  Remainder < h^11/11! <
-// 2e-10 relative at the interval end.
-__device__ __forceinline__ float sin_small_rev(float x) {
-    const float h = 6.28318530717958647692f * x;
-    const float h2 = h * h;
-    float p = 2.75573192e-6f;            //  1/9!
-    p = p * h2 - 1.98412698e-4f;         // -1/7!
-    p = p * h2 + 8.33333333e-3f;         //  1/5!
-    p = p * h2 - 1.66666667e-1f;         // -1/3!
-    return h + h * (h2 * p);
+// sin(2 pi t) and cos(2 pi t) - 1 for |t| <= 1/8 revolution.  Minimax
+// polynomials whose leading coefficients (2 pi, -2 pi^2) are split into float
+// hi + lo parts; the coefficients come from sincos_coeffs.py in this folder.
+// Max error in exact arithmetic: sine 8.8e-11 relative, cosine 1.0e-10
+// absolute, so what remains is the rounding of the last operations, which
+// varies erratically from pair to pair and averages out over the pair sum.
+// The hardware sine and cosine are not used for the recurrence seeds: their
+// error, though below 1.3e-7, follows the phase smoothly (it includes a
+// relative scale error of -3.2e-8 on both), so it is shared by all near
+// neighbours of similar length, and the recurrence carries an error in
+// delta_1 forward growing with the step count.  At the deep minima of I(q)
+// that coherent error is amplified several hundred times.
+constexpr float kS1_HI = 6.283185482e+00f;
+constexpr float kS1_LO = -1.748455531e-07f;
+constexpr float kS3 = -4.134170151e+01f;
+constexpr float kS5 = 8.160494995e+01f;
+constexpr float kS7 = -7.666957855e+01f;
+constexpr float kS9 = 4.052738571e+01f;
+constexpr float kC2_HI = -1.973920822e+01f;
+constexpr float kC2_LO = -5.807431762e-07f;
+constexpr float kC4 = 6.493936157e+01f;
+constexpr float kC6 = -8.544713593e+01f;
+constexpr float kC8 = 5.934757233e+01f;
+
+__device__ __forceinline__ void sincos_poly(float t, float &s, float &cm1) {
+    const float t2 = t * t;
+    float ps = __builtin_fmaf(t2, kS9, kS7);
+    ps = __builtin_fmaf(t2, ps, kS5);
+    ps = __builtin_fmaf(t2, ps, kS3);
+    ps = __builtin_fmaf(t2, ps, kS1_LO);
+    s = __builtin_fmaf(t, kS1_HI, t * ps);
+    float pc = __builtin_fmaf(t2, kC8, kC6);
+    pc = __builtin_fmaf(t2, pc, kC4);
+    pc = __builtin_fmaf(t2, pc, kC2_LO);
+    cm1 = __builtin_fmaf(t2, pc, t2 * kC2_HI);
 }
 
-// 1/sqrt(r2) with one Newton step on the hardware estimate: the estimate's
-// relative error (and its bias) is squared away, leaving the rounding of the
-// last operations.  A bias in r acts as a q-scale error on every pair at once,
-// which the cancellation at deep minima of I(q) would amplify.
-__device__ __forceinline__ float rsqrt_refined(float r2) {
+// sin(2 pi x) and cos(2 pi x) for a phase x in revolutions, |x| < 2^22.  The
+// reduction to the nearest quarter turn is exact, and the result is odd
+// (sine) and even (cosine) in x bit for bit.
+__device__ __forceinline__ void sincos_rev(float x, float &s, float &c) {
+    const float k = __builtin_rintf(4.0f * x);
+    const float t = __builtin_fmaf(k, -0.25f, x);
+    float st, cm1;
+    sincos_poly(t, st, cm1);
+    const float ct = 1.0f + cm1;
+    const int qd = (int)k;
+    const bool swap = (qd & 1) != 0;
+    const float a = swap ? ct : st;
+    const float b = swap ? st : ct;
+    s = __int_as_float(__float_as_int(a) ^ ((qd & 2) << 30));
+    c = __int_as_float(__float_as_int(b) ^ (((qd + 1) & 2) << 30));
+}
+
+// r = sqrt(r2) and rinv = 1/r, each within about half an ulp.  The hardware
+// reciprocal-square-root estimate has an error that is a fixed function of the
+// mantissa of r2: it is shared by every pair of about the same length, so it
+// does not average out over pairs, and at a deep minimum of I(q) -- a small
+// difference of the large same-sign sum over near neighbours and the sum over
+// distant pairs -- it is amplified by the ratio of the two.  A plain Newton
+// step y (1.5 - r2 y^2 / 2) does not remove it: the factor in parentheses is
+// rounded at the coarse ulp of 1.  Here the corrections are formed from exact
+// FMA residuals instead (r2 - r0^2, 1 - r y), which are small numbers whose
+// rounding is negligible.
+__device__ __forceinline__ void sqrt_and_rsqrt(float r2, float &r, float &rinv) {
     const float y = __builtin_amdgcn_rsqf(r2);
-    const float hx = 0.5f * r2;
-    return y * __builtin_fmaf(-hx * y, y, 1.5f);
+    const float r0 = r2 * y;
+    const float d = __builtin_fmaf(-r0, r0, r2);
+    r = __builtin_fmaf(d, 0.5f * y, r0);
+    const float e = __builtin_fmaf(-r, y, 1.0f);
+    rinv = __builtin_fmaf(y, e, y);
 }
 
 // FP32 seed: q0rev = q_s / 2pi split as q0rev + q0lo (revolutions per A),
@@ -184,13 +231,21 @@ __device__ __forceinline__ float rsqrt_refined(float r2) {
 __device__ __forceinline__ Seed32 seed_f32(float r2, float q0rev, float q0lo, float hrev,
                                            float take, bool small_h) {
     r2 = fmaxf(r2, 1e-12f);  // coincident atoms: the r -> 0 limit s_k -> q_k
-    const float rinv = rsqrt_refined(r2);
-    const float r = r2 * rinv;
+    float r, rinv;
+    sqrt_and_rsqrt(r2, r, rinv);
     const float arev = __builtin_fmaf(q0rev, r, q0lo * r);
     const float hr = hrev * r;
-    const float sa = __builtin_amdgcn_sinf(__builtin_amdgcn_fractf(arev));
-    const float cah = __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(arev + hr));
-    const float sh = small_h ? sin_small_rev(hr) : __builtin_amdgcn_sinf(__builtin_amdgcn_fractf(hr));
+    float sa, ca, sh, chm1;
+    sincos_rev(arev, sa, ca);
+    if (small_h) {
+        sincos_poly(hr, sh, chm1);
+    } else {
+        float ch;
+        sincos_rev(hr, sh, ch);
+        chm1 = ch - 1.0f;
+    }
+    // cos(a + h) = cos a + (cos a (cos h - 1) - sin a sin h)
+    const float cah = ca + __builtin_fmaf(ca, chm1, -sa * sh);
     const float amp = take * rinv;
     Seed32 z;
     z.s = sa * amp;
@@ -354,8 +409,8 @@ __device__ __forceinline__ void pair_block(const Atom<T> *__restrict__ atoms, in
         } else {
             if constexpr (sizeof(T) == 4) {
                 const float rr2 = fmaxf(r2, 1e-12f);
-                const float rinv = rsqrt_refined(rr2);
-                const float r = rr2 * rinv;
+                float r, rinv;
+                sqrt_and_rsqrt(rr2, r, rinv);
                 const float amp = take * rinv;
 #pragma unroll
                 for (int k = 0; k < Q; ++k) {

@@ -114,12 +114,12 @@ def test_server_alone_equals_in_mix_and_admission(engine, foxs_engine):
         total = m.intensity(1.05, 0.5)[0]
         assert np.all(np.isfinite(total))
     # admission: a budget smaller than one request fails that request only
-    with Server(engine, "fp32", memory_budget=2_000_000) as srv:
+    with Server(engine, "fp32", memory_budget=32_000_000) as srv:
         small = srv.submit(inps[0])
         m, _ = srv.wait(small)
         assert m.nq == len(inps[0].q)
-        big = foxs_engine.prepare_workload(
-            Workload("big", *globule(20000, radius_for(20000), 77), linear_q(0, 0.5, 800)))
+        bnames, bpos, bw = globule(20000, radius_for(20000), 77)
+        big = foxs_engine.prepare_workload(Workload("big", bnames, bpos, linear_q(0, 0.5, 800), bw))
         rid = srv.submit(big)
         with pytest.raises(OutOfDeviceMemoryError):
             srv.wait(rid)
