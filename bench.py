@@ -33,6 +33,11 @@ UNIT = "pair·q terms/s"
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
 FLOP_PER_TERM_SOLVENT = 14.0
+# HBM bytes per launch of the pair kernel on the default solvent10k plan (Q = 64,
+# span = 448, 14,760 items), from rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
+# (scale factors from a plain-copy calibration); profiles/debye_tiled_f32_solvent10k.md
+TRAFFIC_SOLVENT10K = {"q_per_lane": 64, "span": 448, "work_items": 14760,
+                      "bytes": (2379 * 2.0 + 44805 * 1.0) * 1024}
 PAIRS = ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))
 
 
@@ -72,6 +77,10 @@ def main():
                     help="batch512: structures timed on the sequential plain FP64 path")
     ap.add_argument("--rates", default="10,50,100,200,500,1000,2000")
     ap.add_argument("--requests", type=int, default=100, help="served: requests per rate")
+    ap.add_argument("--served-rate", type=float, default=200.0,
+                    help="default run: offered rate of its short served stream")
+    ap.add_argument("--served-requests", type=int, default=200,
+                    help="default run: requests in its short served stream (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
@@ -161,6 +170,15 @@ def main():
         log(f"cpu baseline: {cpu['value']:.3e} terms/s on {threads} threads ({tcpu:.1f} s); "
             f"fp32 vs oracle {err_oracle:.2e}")
 
+    plan = {"q_per_lane": st.q_per_lane, "span": st.span, "work_items": st.work_items}
+    traffic = (TRAFFIC_SOLVENT10K["bytes"]
+               if all(TRAFFIC_SOLVENT10K[k] == v for k, v in plan.items()) else None)
+
+    served = None
+    if args.served_requests > 0:
+        rec = _served_streams(eng, p, [args.served_rate], args.served_requests)[0]
+        served = {"workload": SERVED_WORKLOAD, **rec}
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -180,12 +198,13 @@ def main():
                    "precision": "fp32_compensated", "q_per_lane": st.q_per_lane,
                    "block": st.block, "span": st.span, "work_items": st.work_items},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": "debye_tiled_kernel<float,%d,true>" % st.q_per_lane,
                      "kernel_ms": pair_ms, "flop_per_term": FLOP_PER_TERM_SOLVENT},
         "cpu_baseline": cpu,
         "max_rel_err_vs_fp64_gpu": err_fp64,
         "max_rel_err_vs_oracle_sampled_q": err_oracle,
+        "served": served,
         "extra": extra,
     }
     print(json.dumps(out), flush=True)
@@ -364,15 +383,18 @@ def _request_job(job):
     return globule(n, radius_for(n), seed, solvent=True)
 
 
-def bench_served(args):
-    """Poisson request streams at several rates through the serving loop."""
+def _served_streams(eng, ff, rates, requests):
+    """Poisson request streams through the serving loop, one stream per rate.
+
+    Each request is a fresh seeded globule (N log-uniform in [300, 30000], nq
+    uniform in [200, 800], q on [0, 0.5], solvent on) followed by five
+    intensity() calls at random knobs.  Latency runs from arrival to the last
+    of those calls; one client thread per in-flight request does the waiting."""
     import numpy as np
-    from concurrent.futures import ProcessPoolExecutor
+    from concurrent.futures import ProcessPoolExecutor, ThreadPoolExecutor
     from pripps_amd.server import Server
     from pripps_amd.synth import Workload, linear_q, request_stream
-    eng, ff = _engine_and_ff()
-    rates = [float(r) for r in args.rates.split(",")]
-    total = args.requests * len(rates)
+    total = requests * len(rates)
     ns, nqs, knobs, seeds = request_stream(total)
     t = time.perf_counter()
     with ProcessPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
@@ -384,38 +406,54 @@ def bench_served(args):
         f"mean {terms.mean():.3e} terms/request")
     rng = np.random.default_rng(11)
     results = []
-    with Server(eng, "fp32") as srv:
-        # warm both lanes
-        for i in range(2):
+
+    def finish(srv, i, rid, ts):
+        m, build_ms = srv.wait(rid)
+        for vs, cd in knobs[i]:
+            m.intensity(vs, cd)
+        done = time.perf_counter()
+        m.close()
+        return (done - ts) * 1e3, build_ms, done
+
+    with Server(eng, "fp32") as srv, ThreadPoolExecutor(max_workers=64) as clients:
+        for i in range(2):  # warm both lanes
             srv.wait(srv.submit(inps[i]))
         for ri, rate in enumerate(rates):
-            idx = list(range(ri * args.requests, (ri + 1) * args.requests))
+            idx = list(range(ri * requests, (ri + 1) * requests))
             arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(idx)))
             t0 = time.perf_counter()
-            rids = []
+            futs = []
             for k, i in enumerate(idx):
                 delay = t0 + arrivals[k] - time.perf_counter()
                 if delay > 0:
                     time.sleep(delay)
-                rids.append((i, srv.submit(inps[i]), time.perf_counter()))
-            lat = []
-            for i, rid, ts in rids:
-                m, build_lat = srv.wait(rid)
-                for vs, cd in knobs[i]:
-                    m.intensity(vs, cd)
-                lat.append((time.perf_counter() - ts) * 1e3)
-                m.close()
-            t1 = time.perf_counter()
-            lat = np.array(lat)
+                ts = time.perf_counter()
+                futs.append(clients.submit(finish, srv, i, srv.submit(inps[i]), ts))
+            res = [f.result() for f in futs]
+            t1 = max(r[2] for r in res)
+            lat = np.array([r[0] for r in res])
+            build = np.array([r[1] for r in res])
             rec = {"offered_rps": rate, "requests": len(idx), "wall_s": t1 - t0,
                    "completed_rps": len(idx) / (t1 - t0), "terms_per_s": float(terms[idx].sum() / (t1 - t0)),
-                   "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))}
+                   "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                   "build_p50_ms": float(np.percentile(build, 50)),
+                   "build_p99_ms": float(np.percentile(build, 99))}
             results.append(rec)
             log(f"served rate {rate}: {rec}")
-    print(json.dumps({"workload": "served: Poisson request streams, globules N log-U[300,30000], "
-                                  "nq U[200,800], q [0,0.5], solvent, +5 intensity() per request",
-                      "latency": "submit to model ready plus 5 intensity() calls",
-                      "rates": results}), flush=True)
+    return results
+
+
+SERVED_WORKLOAD = ("served: Poisson request streams, globules N log-U[300,30000], nq U[200,800], "
+                   "q [0,0.5], solvent, +5 intensity() per request; latency = arrival to the last "
+                   "intensity()")
+
+
+def bench_served(args):
+    """Poisson request streams at several rates through the serving loop."""
+    eng, ff = _engine_and_ff()
+    rates = [float(r) for r in args.rates.split(",")]
+    results = _served_streams(eng, ff, rates, args.requests)
+    print(json.dumps({"workload": SERVED_WORKLOAD, "rates": results}), flush=True)
 
 
 WORKLOADS = {"vacuum1k": bench_vacuum1k, "helix60k": bench_helix60k, "batch512": bench_batch512,
