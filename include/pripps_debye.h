@@ -107,6 +107,10 @@ typedef struct {
     double pair_q_terms;         /* N(N-1)/2 * nq */
     int32_t q_per_lane, block, span;
     int64_t work_items;
+    int32_t frames;              /* frames averaged (pripps_debye_frames_average), else 0 */
+    int32_t frames_per_launch;   /* frames per persistent launch */
+    int32_t graph_launches;      /* chunks submitted by replaying the captured HIP graph */
+    int32_t reserved;
 } pripps_build_stats;
 
 /* Thread-local message of the last failed call on this thread ("" if none). */
@@ -195,6 +199,69 @@ int pripps_debye_model_stats(const pripps_debye_model *model, pripps_build_stats
 int pripps_debye_model_recompute(pripps_debye_model *model, int32_t repeats,
                                  double *pair_ms_mean, double *total_ms);
 int pripps_debye_model_free(pripps_debye_model *model);
+
+/* ---------------------------------------------------------------------- */
+/* Autotuning.  No reference counterpart (the reference has no kernels).   */
+/* ---------------------------------------------------------------------- */
+/* One entry of an engine's tuning table: the plan used for structures of
+ * about 2^log2_atoms atoms and nq <= 2^log2_nq at one precision and q-grid
+ * kind.  span_shift scales the default j-span by 2^span_shift. */
+typedef struct {
+    int32_t precision;
+    int32_t uniform;       /* 1: uniform q grid (angle-addition recurrence) */
+    int32_t log2_atoms;    /* round(log2(n_atoms)) */
+    int32_t log2_nq;       /* ceil(log2(nq)) */
+    int32_t q_per_lane;
+    int32_t block;
+    int32_t span_shift;
+    int32_t variant;
+    double pair_ms;        /* measured pair-kernel time of the tuned input */
+} pripps_tuning_entry;
+
+/*
+ * Time candidate plans (q points per lane, j-span, pair block, inner-loop
+ * variant) on `input` by coordinate descent, `repeats` timed passes each,
+ * and record the fastest in the engine's tuning table.  Later builds whose
+ * `tuning` is NULL (or all zero) take the table entry of their size class;
+ * without one they use the built-in defaults.  `best` (may be NULL) receives
+ * the winning plan in absolute terms for this input.  A structure's plan
+ * depends only on the structure and the table, so its profiles stay the same
+ * alone and in a batch.
+ */
+int pripps_engine_autotune(pripps_engine *engine, const pripps_debye_input *input,
+                           int32_t precision, int32_t repeats, pripps_tuning *best,
+                           double *best_pair_ms);
+/* Copy up to `capacity` table entries to `entries`; *count = entries held. */
+int pripps_engine_tuning_table(pripps_engine *engine, pripps_tuning_entry *entries,
+                               int32_t capacity, int32_t *count);
+/* Insert or replace the entry of one size class (e.g. a saved table). */
+int pripps_engine_set_tuning_entry(pripps_engine *engine, const pripps_tuning_entry *entry);
+
+/* ---------------------------------------------------------------------- */
+/* Trajectory frames.                                                      */
+/* ---------------------------------------------------------------------- */
+/*
+ * Weighted average of the six profiles over the frames of a trajectory:
+ * trajectory_average (src/trajectory.rs:167-251) with the stride / weight
+ * semantics of XtcIterator (src/trajectory.rs:29-135).  `topology` gives the
+ * kinds, q grid, form-factor tables and hydration weights; its x/y/z are
+ * not read.  `coords` holds [n_frames][n_atoms][3] positions in Angstrom.
+ * Frames 0, stride, 2 stride, ... are used; `frame_weights` ([n_frames],
+ * indexed by the original frame number, NULL = all ones) must have a finite
+ * non-zero sum, and so must the weights of the frames used
+ * (PRIPPS_ERR_VALIDATION otherwise).  The result's profiles are
+ * sum_f w_f P_f / sum_f w_f, so its intensity(vs, cd) is the weighted average
+ * of the per-frame intensities.  Frames run in chunks of up to
+ * max_frames_per_launch (0 = as many as a 4 GiB workspace holds) through the
+ * batched persistent launch; the first full chunk's submission (upload, pair
+ * kernels, reduction, weighted accumulation) is captured as a HIP graph and
+ * every further full chunk replays it.  Precision: FP32_COMPENSATED or FP64.
+ */
+int pripps_debye_frames_average(pripps_engine *engine, const pripps_debye_input *topology,
+                                int32_t n_frames, const double *coords,
+                                const double *frame_weights, int32_t stride, int32_t precision,
+                                const pripps_tuning *tuning, int32_t max_frames_per_launch,
+                                pripps_debye_model **out);
 
 /* ---------------------------------------------------------------------- */
 /* Serving: continuous batching with admission by free device memory.      */

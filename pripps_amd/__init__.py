@@ -23,6 +23,8 @@ SASA tessellation), and ``precision`` ("fp32" compensated, "fp64", or
 from __future__ import annotations
 
 import enum
+import os
+import re
 from dataclasses import dataclass
 from typing import Optional, Sequence, Union
 
@@ -208,6 +210,63 @@ class Pripps:
         native = self.engine.build(inp, precision, tuning)
         return DebyeModel(native, q)
 
+    def trajectory_average(self, structure_file, frames, *, qmin: float = 0.0, qmax: float = 0.5,
+                           nq: int = 100, excluded=None, hydration=None, weights=None,
+                           stride: int = 1, units: str = "angstrom",
+                           bulk_electron_density: float = 0.334, q_values=None,
+                           hydration_weights=None, precision: str = "fp32",
+                           tuning: Optional[_native.Tuning] = None,
+                           max_frames_per_launch: int = 0) -> DebyeModel:
+        """Weighted average of the Debye profiles over trajectory frames: the
+        reference's ``trajectory_average`` (src/trajectory.rs:167-251) for the
+        Debye scheme, with ``XtcIterator``'s stride and per-frame weights
+        (src/trajectory.rs:29-135).  ``structure_file`` is the template (atom
+        names and count); ``frames`` is an array [n_frames][n_atoms][3] in
+        ``units`` ("angstrom" or "nm", the XTC unit the reference converts
+        with a factor 10).  ``weights`` is an array with one weight per frame
+        or the path of a weights file (one number per line), indexed by the
+        original frame number.  The returned model's ``intensity(vs, cd)`` is
+        the weighted average of the per-frame intensities."""
+        st = structure_from_file(structure_file, self.formfactors.kinds())
+        return self.trajectory_average_from_arrays(
+            st.ids, frames, qmin=qmin, qmax=qmax, nq=nq, excluded=excluded, hydration=hydration,
+            weights=weights, stride=stride, units=units,
+            bulk_electron_density=bulk_electron_density, q_values=q_values,
+            hydration_weights=hydration_weights, precision=precision, tuning=tuning,
+            max_frames_per_launch=max_frames_per_launch)
+
+    def trajectory_average_from_arrays(self, ids, frames, *, qmin=0.0, qmax=0.5, nq=100,
+                                       excluded=None, hydration=None, weights=None, stride=1,
+                                       units="angstrom", bulk_electron_density=0.334,
+                                       q_values=None, hydration_weights=None, precision="fp32",
+                                       tuning=None, max_frames_per_launch=0) -> DebyeModel:
+        if units not in ("angstrom", "nm"):
+            raise ValueError(f"unknown units: {units} (expected `angstrom` or `nm`)")
+        ids = np.asarray(ids, dtype=np.int64)
+        f = np.asarray(frames, dtype=np.float64)
+        if f.ndim != 3 or f.shape[1:] != (len(ids), 3):
+            raise ValidationError(f"number of atoms in structure ({len(ids)}) does not match "
+                                  f"trajectory frames of shape {list(f.shape)}")
+        if units == "nm":
+            f = f * 10.0
+        if isinstance(weights, (str, os.PathLike)):
+            weights = read_weights(weights)
+        if q_values is None:
+            nq = int(nq)
+            if nq < 0:
+                raise ValidationError("nq must be >= 0")
+            q = np.array([qmin + (qmax - qmin) * i / float(max(nq - 1, 1)) for i in range(nq)])
+        else:
+            q = np.ascontiguousarray(q_values, dtype=np.float64)
+        pos0 = f[0] if len(f) else np.zeros((len(ids), 3))
+        inp = self.prepare_input(pos0, ids, q, excluded=excluded, hydration=hydration,
+                                 hydration_weights=hydration_weights,
+                                 bulk_electron_density=bulk_electron_density)
+        native = self.engine.frames_average(inp, f, weights=weights, stride=stride,
+                                            precision=precision, tuning=tuning,
+                                            max_frames_per_launch=max_frames_per_launch)
+        return DebyeModel(native, q)
+
     def names_to_ids(self, names: Sequence[str]) -> np.ndarray:
         return resolve_ids(names, self.formfactors.kinds())
 
@@ -220,9 +279,40 @@ class Pripps:
                                   bulk_electron_density=bulk_electron_density)
 
 
+_RUST_F64 = re.compile(r"[+-]?(inf|infinity|nan|(\d+\.?\d*|\.\d+)([eE][+-]?\d+)?)",
+                       re.IGNORECASE)
+
+
+def read_weights(path) -> np.ndarray:
+    """Per-frame weights, one number per line (reference ``read_weights``,
+    src/trajectory.rs:117-135): the whole line must be a number; the weights
+    are returned as they are (not normalised) and must have a finite non-zero
+    sum."""
+    try:
+        with open(path, "r", encoding="utf-8") as fh:
+            text = fh.read()
+    except OSError as exc:
+        raise FileError(f"{path}: {exc.strerror or exc}") from exc
+    lines = text.split("\n")
+    if lines and lines[-1] == "":
+        lines.pop()
+    out = []
+    for line in lines:
+        if line.endswith("\r"):
+            line = line[:-1]
+        if not _RUST_F64.fullmatch(line):
+            raise ValidationError(f"weight parse error: invalid float literal ({line!r})")
+        out.append(float(line))
+    w = np.array(out, dtype=np.float64)
+    total = float(np.sum(w)) if len(w) else 0.0
+    if not (np.isfinite(total) and abs(total) >= np.finfo(np.float64).tiny):
+        raise ValidationError("sum of weights is zero or infinite")
+    return w
+
+
 __all__ = [
     "Pripps", "DebyeModel", "Intensity", "FitResult", "Excluded", "Hydration", "FitMethod",
-    "Engine", "DEFAULT_C1_BOUNDS", "DEFAULT_C2_BOUNDS", "PrippsError", "FileError", "YamlError",
+    "Engine", "read_weights", "DEFAULT_C1_BOUNDS", "DEFAULT_C2_BOUNDS", "PrippsError", "FileError", "YamlError",
     "UnknownAtomError", "UndefinedFormFactorError", "QOutOfRangeError", "ValidationError",
     "UsageError", "InternalError", "DeviceError", "OutOfDeviceMemoryError",
 ]

@@ -60,7 +60,17 @@ class BuildStats(ctypes.Structure):
     _fields_ = [("kernel_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
                 ("pair_q_terms", ctypes.c_double), ("q_per_lane", ctypes.c_int32),
                 ("block", ctypes.c_int32), ("span", ctypes.c_int32),
-                ("work_items", ctypes.c_int64)]
+                ("work_items", ctypes.c_int64), ("frames", ctypes.c_int32),
+                ("frames_per_launch", ctypes.c_int32), ("graph_launches", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+class TuningEntry(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int32), ("uniform", ctypes.c_int32),
+                ("log2_atoms", ctypes.c_int32), ("log2_nq", ctypes.c_int32),
+                ("q_per_lane", ctypes.c_int32), ("block", ctypes.c_int32),
+                ("span_shift", ctypes.c_int32), ("variant", ctypes.c_int32),
+                ("pair_ms", ctypes.c_double)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -75,8 +85,9 @@ EXPORTS = (
     "pripps_debye_model_nq", "pripps_debye_model_q_values", "pripps_debye_model_intensity",
     "pripps_debye_model_profiles", "pripps_debye_model_fit", "pripps_debye_model_from_profiles",
     "pripps_debye_model_stats", "pripps_debye_model_recompute",
-    "pripps_debye_model_free", "pripps_server_create", "pripps_server_submit",
-    "pripps_server_wait", "pripps_server_destroy",
+    "pripps_debye_model_free", "pripps_engine_autotune", "pripps_engine_tuning_table",
+    "pripps_engine_set_tuning_entry", "pripps_debye_frames_average", "pripps_server_create",
+    "pripps_server_submit", "pripps_server_wait", "pripps_server_destroy",
 )
 
 _lib = None
@@ -120,6 +131,16 @@ def load():
                                                    ctypes.POINTER(ctypes.c_double)]
         L.pripps_debye_model_stats.argtypes = [vp, ctypes.POINTER(BuildStats)]
         L.pripps_debye_model_free.argtypes = [vp]
+        L.pripps_engine_autotune.argtypes = [vp, ctypes.POINTER(DebyeInput), ctypes.c_int32,
+                                             ctypes.c_int32, ctypes.POINTER(Tuning),
+                                             ctypes.POINTER(ctypes.c_double)]
+        L.pripps_engine_tuning_table.argtypes = [vp, ctypes.POINTER(TuningEntry), ctypes.c_int32,
+                                                 ctypes.POINTER(ctypes.c_int32)]
+        L.pripps_engine_set_tuning_entry.argtypes = [vp, ctypes.POINTER(TuningEntry)]
+        L.pripps_debye_frames_average.argtypes = [vp, ctypes.POINTER(DebyeInput), ctypes.c_int32,
+                                                  _D, _D, ctypes.c_int32, ctypes.c_int32,
+                                                  ctypes.POINTER(Tuning), ctypes.c_int32,
+                                                  ctypes.POINTER(vp)]
         L.pripps_server_create.argtypes = [vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
         L.pripps_server_submit.argtypes = [vp, ctypes.POINTER(DebyeInput),
                                            ctypes.POINTER(ctypes.c_int64)]

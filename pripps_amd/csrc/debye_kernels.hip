@@ -503,7 +503,7 @@ __global__ void __launch_bounds__(256) debye_tiled_kernel(TiledArgs args) {
         int c = 0;
         while (j < it.j1) {
             while (sd.cls_start[c + 1] <= j) ++c;
-            const int jend = min(min(j + args.block, it.j1), sd.cls_start[c + 1]);
+            const int jend = min(min(j + sd.block, it.j1), sd.cls_start[c + 1]);
             const int lk = sd.cls_kind[c];
             const bool wet = sd.cls_wet[c] != 0;
             const bool mask = j <= it.i0 + it.icount - 1;  // chunk meets the diagonal
@@ -773,6 +773,29 @@ __global__ void fill_int_kernel(int *p, int n, int v) {
     if (i < n) p[i] = v;
 }
 
+__global__ void fill_double_kernel(double *p, int64_t n, double v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Weighted sum of the profiles of a chunk of trajectory frames into the
+// running FP64 accumulator, slots in a fixed order (deterministic).
+__global__ void frames_accumulate_kernel(const double *__restrict__ prof, int n_slots,
+                                         const double *__restrict__ w, int64_t n,
+                                         double *__restrict__ acc) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double a = acc[i];
+    for (int c = 0; c < n_slots; ++c) a += w[c] * prof[(int64_t)c * n + i];
+    acc[i] = a;
+}
+
+__global__ void frames_normalize_kernel(const double *__restrict__ acc, int64_t n, double wsum,
+                                        double *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = acc[i] / wsum;
+}
+
 // I(q) = AA + vs^2 EE + cd^2 HH - 2 vs AE + 2 cd AH - 2 vs cd EH, evaluated in
 // the reference's operation order without FMA contraction
 // (src/debye.rs:186-195).  out: [4][nq] total, atoms, excluded, hydration.
@@ -871,6 +894,30 @@ hipError_t launch_combine(const double *prof, int nq, double vs, double cd, doub
                           hipStream_t st) {
     hipLaunchKernelGGL(combine_kernel, dim3((nq + 255) / 256), dim3(256), 0, st, prof, nq, vs, cd,
                        out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_double(double *p, int64_t n, double v, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_double_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_frames_accumulate(const double *prof, int n_slots, const double *w, int nq,
+                                    double *acc, hipStream_t st) {
+    const int64_t n = (int64_t)kProfiles * nq;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(frames_accumulate_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                       prof, n_slots, w, n, acc);
+    return hipGetLastError();
+}
+
+hipError_t launch_frames_normalize(const double *acc, int nq, double wsum, double *out,
+                                   hipStream_t st) {
+    const int64_t n = (int64_t)kProfiles * nq;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(frames_normalize_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                       acc, n, wsum, out);
     return hipGetLastError();
 }
 

@@ -100,6 +100,9 @@ struct pripps_engine {
     uint64_t build_serial = 0;
     bool last_plain = false;
     std::vector<LaunchRecord> last_launches;
+    // autotuned plans per size class (pripps_engine_autotune)
+    std::mutex tune_mu;
+    std::vector<pripps_tuning_entry> tune_table;
 };
 
 struct pripps_debye_model {
@@ -382,26 +385,30 @@ struct Layout {
 // ---------------------------------------------------------------------------
 // build
 // ---------------------------------------------------------------------------
-static int choose_q(const pripps_tuning *tune, int precision, bool uniform, int nq) {
-    int Q = tune && tune->q_per_lane ? tune->q_per_lane : 0;
-    if (precision == PRIPPS_FP64) {
-        if (Q != 0 && Q != 32 && Q != 64) Q = 0;
-        if (Q == 64 && !uniform) Q = 32;
-        return Q ? Q : 32;
-    }
-    if (Q != 0 && Q != 32 && Q != 64 && Q != 128) Q = 0;
-    if (Q == 128 && !uniform) Q = 64;
-    if (Q) return Q;
-    if (!uniform) return 32;
+// ---------------------------------------------------------------------------
+// per-structure plan: q points per lane, j-span, pair block, inner-loop variant
+// ---------------------------------------------------------------------------
+namespace {
+
+struct StructPlan {
+    int Q = 64, span = 256, block = 64, variant = 3;
+};
+
+int valid_q(int precision, bool uniform, int Q) {
+    if (precision == PRIPPS_FP64) return (Q == 32 || (Q == 64 && uniform)) ? Q : 0;
+    return (Q == 32 || Q == 64 || (Q == 128 && uniform)) ? Q : 0;
+}
+
+int default_q(int precision, bool uniform, int nq) {
+    if (precision == PRIPPS_FP64 || !uniform) return 32;
     return nq <= 32 ? 32 : 64;
 }
 
-// j atoms per work item of one structure: about 8 items per wave slot of the
-// GPU when the structure runs alone, 256..8192, a multiple of 64.  It depends
-// on the structure only, so a structure is cut into the same items alone or in
-// any batch, and its profiles are bitwise the same either way.
-static int choose_span(const pripps_tuning *tune, const Prepared &p, int Q, int n_cu) {
-    if (tune && tune->span > 0) return std::max(64, tune->span);
+// j atoms per work item: about 8 items per wave slot of the GPU when the
+// structure runs alone, 256..8192, a multiple of 64.  It depends on the
+// structure only, so a structure is cut into the same items alone or in any
+// batch, and its profiles are bitwise the same either way.
+int default_span(const Prepared &p, int Q, int n_cu) {
     const double strips = (double)((p.nq + Q - 1) / Q);
     const double work = (double)p.tiles.size() * 0.5 * p.n * strips;
     const double slots = std::max(1, n_cu) * 8.0;
@@ -409,11 +416,32 @@ static int choose_span(const pripps_tuning *tune, const Prepared &p, int Q, int 
     return std::max(256, std::min(8192, (span + 63) / 64 * 64));
 }
 
-namespace {
-// Structures of one kernel shape (Q, uniform grid) share one persistent launch.
+int shifted_span(int base, int shift) {
+    double s = std::ldexp((double)base, shift);
+    s = std::max(64.0, std::min(16384.0, s));
+    return ((int)s + 63) / 64 * 64;
+}
+
+int valid_variant(int precision, bool uniform, int Q, int v) {
+    if (precision != PRIPPS_FP32_COMPENSATED || !uniform) return 3;
+    if ((v == 1 || v == 2) && Q == 64) return v;
+    if (v == 4 && (Q == 32 || Q == 64)) return 4;
+    return 3;
+}
+
+int log2_atoms_of(int64_t n) { return n <= 1 ? 0 : (int)std::lround(std::log2((double)n)); }
+
+int log2_nq_of(int nq) {
+    int b = 0;
+    while ((1 << b) < nq && b < 30) ++b;
+    return b;
+}
+
+// Structures of one kernel shape (Q, uniform grid, variant) share one
+// persistent launch.
 struct LaunchGroup {
     bool uniform = true;
-    int Q = 64;
+    int Q = 64, variant = 3;
     std::vector<int> members;
     std::vector<Item> items;
     std::vector<int32_t> csr_ptr, csr_items, strip_struct, strip_index, stage_ptr;
@@ -421,15 +449,59 @@ struct LaunchGroup {
     size_t o_items = 0, o_csr_ptr = 0, o_csr_items = 0, o_ss = 0, o_si = 0, o_sp = 0;
     size_t o_counter = 0, o_partials = 0, o_stage = 0;
 };
+
+// Everything a submission needs: per-structure plans, launch groups and the
+// arena layout.  [0, upload_bytes) of the arena mirrors the pinned staging
+// buffer and arrives with one copy; the rest is device-only scratch.
+struct BuildPlan {
+    bool fp64 = false, plain = false;
+    int ns = 0;
+    std::vector<StructPlan> sp;
+    std::vector<LaunchGroup> groups;
+    std::vector<size_t> o_atoms, o_ffa, o_ffe, o_ffw, o_kn, o_kw, o_tiles, o_pp;
+    size_t o_structs = 0, o_user = 0, upload_bytes = 0, total_bytes = 0;
+    int64_t n_items = 0;
+};
+
 }  // namespace
 
-static void plan_group(LaunchGroup &g, const std::vector<Prepared> &P, const pripps_tuning *tune,
-                       int n_cu, std::vector<int> &span_of) {
+static StructPlan resolve_plan(pripps_engine *e, const pripps_tuning *tune, int precision,
+                               const Prepared &p) {
+    StructPlan sp;
+    const bool expl = tune && (tune->q_per_lane || tune->block || tune->span || tune->variant);
+    pripps_tuning_entry te{};
+    bool have = false;
+    if (!expl) {
+        std::lock_guard<std::mutex> lk(e->tune_mu);
+        const int la = log2_atoms_of(p.n), ln = log2_nq_of(p.nq);
+        for (const auto &x : e->tune_table)
+            if (x.precision == precision && x.uniform == (p.uniform ? 1 : 0) && x.log2_atoms == la &&
+                x.log2_nq == ln) {
+                te = x;
+                have = true;
+                break;
+            }
+    }
+    int Q = expl ? valid_q(precision, p.uniform, tune->q_per_lane)
+                 : have ? valid_q(precision, p.uniform, te.q_per_lane) : 0;
+    if (!Q) Q = default_q(precision, p.uniform, p.nq);
+    sp.Q = Q;
+    const int base = default_span(p, Q, e->n_cu);
+    if (expl && tune->span > 0) sp.span = std::max(64, tune->span);
+    else if (have) sp.span = shifted_span(base, te.span_shift);
+    else sp.span = base;
+    const int block = expl && tune->block > 0 ? tune->block : (have && te.block > 0 ? te.block : 64);
+    sp.block = std::max(8, std::min(block, 4096));
+    sp.variant = valid_variant(precision, p.uniform, Q, expl ? tune->variant : (have ? te.variant : 3));
+    return sp;
+}
+
+static void plan_group(LaunchGroup &g, const std::vector<Prepared> &P,
+                       const std::vector<StructPlan> &sp) {
     for (int s : g.members) {
         const Prepared &p = P[s];
         const int nst = (p.nq + g.Q - 1) / g.Q;
-        const int span = choose_span(tune, p, g.Q, n_cu);
-        span_of[s] = span;
+        const int span = sp[s].span;
         for (const int4 &t : p.tiles)
             for (int32_t j0 = t.x; j0 < p.n; j0 += span) {
                 const int32_t j1 = std::min<int32_t>(p.n, j0 + span);
@@ -480,6 +552,196 @@ static void plan_group(LaunchGroup &g, const std::vector<Prepared> &P, const pri
         }
 }
 
+// Plan and lay out one submission.  user_bytes reserves a caller region at
+// the end of the uploaded part (B.o_user).
+static void plan_build(pripps_engine *e, const std::vector<Prepared> &P, int precision,
+                       const pripps_tuning *tune, size_t user_bytes, BuildPlan &B) {
+    const int ns = (int)P.size();
+    B.ns = ns;
+    B.fp64 = precision != PRIPPS_FP32_COMPENSATED;
+    B.plain = precision == PRIPPS_FP64_PLAIN;
+    B.sp.assign(ns, StructPlan{});
+    if (!B.plain) {
+        for (int s = 0; s < ns; ++s) {
+            B.sp[s] = resolve_plan(e, tune, precision, P[s]);
+            LaunchGroup *g = nullptr;
+            for (auto &x : B.groups)
+                if (x.Q == B.sp[s].Q && x.uniform == P[s].uniform && x.variant == B.sp[s].variant)
+                    g = &x;
+            if (!g) {
+                B.groups.emplace_back();
+                g = &B.groups.back();
+                g->Q = B.sp[s].Q;
+                g->uniform = P[s].uniform;
+                g->variant = B.sp[s].variant;
+            }
+            g->members.push_back(s);
+        }
+        for (auto &g : B.groups) {
+            plan_group(g, P, B.sp);
+            B.n_items += (int64_t)g.items.size();
+        }
+    }
+    Layout L;
+    const size_t atom_bytes = B.fp64 ? sizeof(Atom<double>) : sizeof(Atom<float>);
+    for (auto *v : {&B.o_atoms, &B.o_ffa, &B.o_ffe, &B.o_ffw, &B.o_kn, &B.o_kw, &B.o_tiles, &B.o_pp})
+        v->assign(ns, 0);
+    for (int s = 0; s < ns; ++s) {
+        B.o_atoms[s] = L.take(std::max<size_t>(1, (size_t)P[s].n) * atom_bytes);
+        B.o_ffa[s] = L.take(P[s].ffa.size() * 8 + 8);
+        B.o_ffe[s] = L.take(P[s].ffe.size() * 8 + 8);
+        B.o_ffw[s] = L.take(P[s].ffw.size() * 8 + 8);
+        B.o_kn[s] = L.take(P[s].K * 8);
+        B.o_kw[s] = L.take(P[s].K * 8);
+        if (B.plain) B.o_tiles[s] = L.take(P[s].tiles.size() * sizeof(int4) + 16);
+    }
+    B.o_structs = L.take(ns * sizeof(DevStruct));
+    for (auto &g : B.groups) {
+        g.o_items = L.take(g.items.size() * sizeof(Item) + 16);
+        g.o_csr_ptr = L.take(g.csr_ptr.size() * 4 + 4);
+        g.o_csr_items = L.take(g.csr_items.size() * 4 + 4);
+        g.o_ss = L.take(g.strip_struct.size() * 4 + 4);
+        g.o_si = L.take(g.strip_index.size() * 4 + 4);
+        g.o_sp = L.take(g.stage_ptr.size() * 4 + 4);
+        g.o_counter = L.take(256);
+    }
+    B.o_user = L.take(std::max<size_t>(user_bytes, 8));
+    B.upload_bytes = L.off;
+    if (B.plain) {
+        for (int s = 0; s < ns; ++s)
+            B.o_pp[s] = L.take(std::max<size_t>(1, P[s].tiles.size() * (size_t)P[s].nq * kProfiles) * 8);
+    }
+    for (auto &g : B.groups) {
+        g.o_partials = L.take(std::max<size_t>(1, g.items.size()) * kProfiles * g.Q * 8);
+        const size_t nch = g.stage_ptr.empty() ? 0 : (size_t)g.stage_ptr.back();
+        g.o_stage = L.take(std::max<size_t>(1, nch) * kProfiles * g.Q * 8);
+    }
+    B.total_bytes = L.off;
+}
+
+// Fill the pinned staging buffer for plan B: atoms (converted), tables,
+// structure records (pointing at the arena, q[s] and prof[s]) and work lists.
+static void stage_build(const BuildPlan &B, const std::vector<Prepared> &P, char *hbase,
+                        char *dbase, const std::vector<const double *> &d_q,
+                        const std::vector<double *> &d_prof) {
+    std::vector<DevStruct> ds(B.ns);
+    for (int s = 0; s < B.ns; ++s) {
+        const Prepared &p = P[s];
+        if (B.fp64) {
+            std::memcpy(hbase + B.o_atoms[s], p.pos.data(), p.pos.size() * 8);
+        } else {
+            float *f = (float *)(hbase + B.o_atoms[s]);
+            for (size_t i = 0; i < p.pos.size(); ++i) f[i] = (float)p.pos[i];
+        }
+        std::memcpy(hbase + B.o_ffa[s], p.ffa.data(), p.ffa.size() * 8);
+        if (!p.ffe.empty()) std::memcpy(hbase + B.o_ffe[s], p.ffe.data(), p.ffe.size() * 8);
+        if (!p.ffw.empty()) std::memcpy(hbase + B.o_ffw[s], p.ffw.data(), p.ffw.size() * 8);
+        std::memcpy(hbase + B.o_kn[s], p.kind_n.data(), p.K * 8);
+        std::memcpy(hbase + B.o_kw[s], p.kind_w.data(), p.K * 8);
+        if (B.plain) std::memcpy(hbase + B.o_tiles[s], p.tiles.data(), p.tiles.size() * sizeof(int4));
+        DevStruct &d = ds[s];
+        std::memset(&d, 0, sizeof(d));
+        d.atoms = dbase + B.o_atoms[s];
+        d.q = d_q[s];
+        d.ffa = (const double *)(dbase + B.o_ffa[s]);
+        d.ffe = p.has_e ? (const double *)(dbase + B.o_ffe[s]) : nullptr;
+        d.ffw = p.has_h ? (const double *)(dbase + B.o_ffw[s]) : nullptr;
+        d.profiles = d_prof[s];
+        d.kind_n = (const double *)(dbase + B.o_kn[s]);
+        d.kind_w = (const double *)(dbase + B.o_kw[s]);
+        d.sum_w2 = p.sum_w2;
+        d.q0 = p.q0;
+        d.dq = p.dq;
+        d.n = p.n;
+        d.nq = p.nq;
+        d.K = p.K;
+        d.uniform = p.uniform ? 1 : 0;
+        d.small_h = p.small_h ? 1 : 0;
+        d.n_classes = p.n_classes;
+        d.n_strips = B.plain ? 0 : (p.nq + B.sp[s].Q - 1) / B.sp[s].Q;
+        d.block = B.sp[s].block;
+        for (int c = 0; c <= p.n_classes; ++c) d.cls_start[c] = p.cls_start[c];
+        for (int c = p.n_classes + 1; c <= kMaxClasses; ++c) d.cls_start[c] = p.n;
+        for (int c = 0; c < p.n_classes; ++c) {
+            d.cls_kind[c] = p.cls_kind[c];
+            d.cls_wet[c] = p.cls_wet[c];
+        }
+    }
+    std::memcpy(hbase + B.o_structs, ds.data(), B.ns * sizeof(DevStruct));
+    auto put = [&](size_t off, const void *src, size_t bytes) {
+        if (bytes) std::memcpy(hbase + off, src, bytes);
+    };
+    for (auto &g : B.groups) {
+        put(g.o_items, g.items.data(), g.items.size() * sizeof(Item));
+        put(g.o_csr_ptr, g.csr_ptr.data(), g.csr_ptr.size() * 4);
+        put(g.o_csr_items, g.csr_items.data(), g.csr_items.size() * 4);
+        put(g.o_ss, g.strip_struct.data(), g.strip_struct.size() * 4);
+        put(g.o_si, g.strip_index.data(), g.strip_index.size() * 4);
+        put(g.o_sp, g.stage_ptr.data(), g.stage_ptr.size() * 4);
+    }
+}
+
+static int make_launches(pripps_engine *e, const BuildPlan &B, char *dbase,
+                         std::vector<LaunchRecord> &launches) {
+    launches.clear();
+    for (auto &g : B.groups) {
+        LaunchRecord r;
+        r.fp64 = B.fp64;
+        r.Q = g.Q;
+        r.uniform = g.uniform;
+        r.counter = (int *)(dbase + g.o_counter);
+        r.n_items = (int64_t)g.items.size();
+        r.n_strips = (int)g.strip_struct.size();
+        TiledArgs &a = r.targs;
+        a.structs = (const DevStruct *)(dbase + B.o_structs);
+        a.items = (const Item *)(dbase + g.o_items);
+        a.n_items = r.n_items;
+        a.counter = r.counter;
+        a.partials = (double *)(dbase + g.o_partials);
+        a.variant = g.variant;
+        int bpc = 1;
+        HIPCHK(tiled_occupancy(B.fp64, g.Q, g.uniform, &bpc));
+        bpc = std::max(1, bpc);
+        const int64_t want = (r.n_items + 3) / 4;
+        r.blocks = (int)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)e->n_cu * bpc));
+        FinalizeArgs &f = r.fargs;
+        f.structs = a.structs;
+        f.csr_ptr = (const int32_t *)(dbase + g.o_csr_ptr);
+        f.csr_items = (const int32_t *)(dbase + g.o_csr_items);
+        f.partials = a.partials;
+        f.strip_struct = (const int32_t *)(dbase + g.o_ss);
+        f.strip_index = (const int32_t *)(dbase + g.o_si);
+        f.stage_ptr = (const int32_t *)(dbase + g.o_sp);
+        f.stage = (double *)(dbase + g.o_stage);
+        f.max_chunks = g.max_chunks;
+        f.Q = g.Q;
+        launches.push_back(r);
+    }
+    return PRIPPS_OK;
+}
+
+// The kernels of one submission (after its upload), on `st`.
+static hipError_t enqueue_kernels(const BuildPlan &B, const std::vector<Prepared> &P, char *dbase,
+                                  const std::vector<LaunchRecord> &launches, hipStream_t st) {
+    hipError_t err = hipSuccess;
+    if (B.plain) {
+        for (int s = 0; s < B.ns && err == hipSuccess; ++s) {
+            if (P[s].nq == 0) continue;
+            PlainArgs a;
+            a.sd = (const DevStruct *)(dbase + B.o_structs) + s;
+            a.tiles = (const int4 *)(dbase + B.o_tiles[s]);
+            a.n_tiles = (int)P[s].tiles.size();
+            a.partials = (double *)(dbase + B.o_pp[s]);
+            err = launch_plain(a, P[s].nq, st);
+        }
+    } else {
+        for (auto &r : launches) {
+            if (err == hipSuccess) err = run_launch(r, st, nullptr, nullptr);
+        }
+    }
+    return err;
+}
+
 static int build_models(pripps_engine *e, int32_t ns, const pripps_debye_input *inputs,
                         int32_t precision, const pripps_tuning *tune, pripps_debye_model **out) {
     const auto t_start = std::chrono::steady_clock::now();
@@ -500,11 +762,6 @@ static int build_models(pripps_engine *e, int32_t ns, const pripps_debye_input *
             return rc;
         }
     }
-    const bool fp64 = precision != PRIPPS_FP32_COMPENSATED;
-    const bool plain = precision == PRIPPS_FP64_PLAIN;
-    int block = tune && tune->block > 0 ? tune->block : 64;
-    block = std::max(8, std::min(block, 4096));
-    const int variant = (tune && tune->variant > 0) ? tune->variant : 3;
 
     std::lock_guard<std::mutex> lock(e->mu);
     HIPCHK(hipSetDevice(e->device));
@@ -533,190 +790,46 @@ static int build_models(pripps_engine *e, int32_t ns, const pripps_debye_input *
         }
     }
 
-    // ---- plan: one launch group per kernel shape ----
-    std::vector<LaunchGroup> groups;
-    std::vector<int> q_of(ns, 0), span_of(ns, 0);
-    if (!plain) {
-        for (int s = 0; s < ns; ++s) {
-            const int Q = choose_q(tune, precision, P[s].uniform, P[s].nq);
-            q_of[s] = Q;
-            LaunchGroup *g = nullptr;
-            for (auto &x : groups)
-                if (x.Q == Q && x.uniform == P[s].uniform) g = &x;
-            if (!g) {
-                groups.emplace_back();
-                g = &groups.back();
-                g->Q = Q;
-                g->uniform = P[s].uniform;
-            }
-            g->members.push_back(s);
-        }
-        for (auto &g : groups) plan_group(g, P, tune, e->n_cu, span_of);
-    }
-
-    // ---- layout of the arena ----
-    Layout L;
-    const size_t atom_bytes = fp64 ? sizeof(Atom<double>) : sizeof(Atom<float>);
-    std::vector<size_t> o_atoms(ns), o_ffa(ns), o_ffe(ns), o_ffw(ns), o_kn(ns), o_kw(ns), o_tiles(ns),
-        o_pp(ns);
-    for (int s = 0; s < ns; ++s) {
-        o_atoms[s] = L.take(std::max<size_t>(1, (size_t)P[s].n) * atom_bytes);
-        o_ffa[s] = L.take(P[s].ffa.size() * 8 + 8);
-        o_ffe[s] = L.take(P[s].ffe.size() * 8 + 8);
-        o_ffw[s] = L.take(P[s].ffw.size() * 8 + 8);
-        o_kn[s] = L.take(P[s].K * 8);
-        o_kw[s] = L.take(P[s].K * 8);
-        if (plain) o_tiles[s] = L.take(P[s].tiles.size() * sizeof(int4) + 16);
-    }
-    const size_t o_structs = L.take(ns * sizeof(DevStruct));
-    for (auto &g : groups) {
-        g.o_items = L.take(g.items.size() * sizeof(Item) + 16);
-        g.o_csr_ptr = L.take(g.csr_ptr.size() * 4 + 4);
-        g.o_csr_items = L.take(g.csr_items.size() * 4 + 4);
-        g.o_ss = L.take(g.strip_struct.size() * 4 + 4);
-        g.o_si = L.take(g.strip_index.size() * 4 + 4);
-        g.o_sp = L.take(g.stage_ptr.size() * 4 + 4);
-        g.o_counter = L.take(256);
-    }
-    const size_t upload_bytes = L.off;
-    if (plain) {
-        for (int s = 0; s < ns; ++s)
-            o_pp[s] = L.take(std::max<size_t>(1, P[s].tiles.size() * (size_t)P[s].nq * kProfiles) * 8);
-    }
-    for (auto &g : groups) {
-        g.o_partials = L.take(std::max<size_t>(1, g.items.size()) * kProfiles * g.Q * 8);
-        const size_t nch = g.stage_ptr.empty() ? 0 : (size_t)g.stage_ptr.back();
-        g.o_stage = L.take(std::max<size_t>(1, nch) * kProfiles * g.Q * 8);
-    }
-    const size_t total_bytes = L.off;
-
-    int rc = ensure_arena(e, total_bytes);
-    if (!rc) rc = ensure_staging(e, upload_bytes);
+    BuildPlan B;
+    plan_build(e, P, precision, tune, 0, B);
+    int rc = ensure_arena(e, B.total_bytes);
+    if (!rc) rc = ensure_staging(e, B.upload_bytes);
     if (rc) {
         cleanup();
         return rc;
     }
     char *dbase = (char *)e->arena;
     char *hbase = (char *)e->staging;
-    std::vector<DevStruct> ds(ns);
+    std::vector<const double *> dq(ns);
+    std::vector<double *> dprof(ns);
     for (int s = 0; s < ns; ++s) {
-        const Prepared &p = P[s];
-        if (fp64) {
-            std::memcpy(hbase + o_atoms[s], p.pos.data(), p.pos.size() * 8);
-        } else {
-            float *f = (float *)(hbase + o_atoms[s]);
-            for (size_t i = 0; i < p.pos.size(); ++i) f[i] = (float)p.pos[i];
-        }
-        std::memcpy(hbase + o_ffa[s], p.ffa.data(), p.ffa.size() * 8);
-        if (!p.ffe.empty()) std::memcpy(hbase + o_ffe[s], p.ffe.data(), p.ffe.size() * 8);
-        if (!p.ffw.empty()) std::memcpy(hbase + o_ffw[s], p.ffw.data(), p.ffw.size() * 8);
-        std::memcpy(hbase + o_kn[s], p.kind_n.data(), p.K * 8);
-        std::memcpy(hbase + o_kw[s], p.kind_w.data(), p.K * 8);
-        if (plain) std::memcpy(hbase + o_tiles[s], p.tiles.data(), p.tiles.size() * sizeof(int4));
-        DevStruct &d = ds[s];
-        std::memset(&d, 0, sizeof(d));
-        d.atoms = dbase + o_atoms[s];
-        d.q = models[s]->d_q();
-        d.ffa = (const double *)(dbase + o_ffa[s]);
-        d.ffe = p.has_e ? (const double *)(dbase + o_ffe[s]) : nullptr;
-        d.ffw = p.has_h ? (const double *)(dbase + o_ffw[s]) : nullptr;
-        d.profiles = models[s]->d_prof();
-        d.kind_n = (const double *)(dbase + o_kn[s]);
-        d.kind_w = (const double *)(dbase + o_kw[s]);
-        d.sum_w2 = p.sum_w2;
-        d.q0 = p.q0;
-        d.dq = p.dq;
-        d.n = p.n;
-        d.nq = p.nq;
-        d.K = p.K;
-        d.uniform = p.uniform ? 1 : 0;
-        d.small_h = p.small_h ? 1 : 0;
-        d.n_classes = p.n_classes;
-        d.n_strips = q_of[s] ? (p.nq + q_of[s] - 1) / q_of[s] : 0;
-        for (int c = 0; c <= p.n_classes; ++c) d.cls_start[c] = p.cls_start[c];
-        for (int c = p.n_classes + 1; c <= kMaxClasses; ++c) d.cls_start[c] = p.n;
-        for (int c = 0; c < p.n_classes; ++c) {
-            d.cls_kind[c] = p.cls_kind[c];
-            d.cls_wet[c] = p.cls_wet[c];
-        }
+        dq[s] = models[s]->d_q();
+        dprof[s] = models[s]->d_prof();
     }
-    std::memcpy(hbase + o_structs, ds.data(), ns * sizeof(DevStruct));
-    auto put = [&](size_t off, const void *src, size_t bytes) {
-        if (bytes) std::memcpy(hbase + off, src, bytes);
-    };
-    for (auto &g : groups) {
-        put(g.o_items, g.items.data(), g.items.size() * sizeof(Item));
-        put(g.o_csr_ptr, g.csr_ptr.data(), g.csr_ptr.size() * 4);
-        put(g.o_csr_items, g.csr_items.data(), g.csr_items.size() * 4);
-        put(g.o_ss, g.strip_struct.data(), g.strip_struct.size() * 4);
-        put(g.o_si, g.strip_index.data(), g.strip_index.size() * 4);
-        put(g.o_sp, g.stage_ptr.data(), g.stage_ptr.size() * 4);
-    }
-
+    stage_build(B, P, hbase, dbase, dq, dprof);
     std::vector<LaunchRecord> launches;
-    for (auto &g : groups) {
-        LaunchRecord r;
-        r.fp64 = fp64;
-        r.Q = g.Q;
-        r.uniform = g.uniform;
-        r.counter = (int *)(dbase + g.o_counter);
-        r.n_items = (int64_t)g.items.size();
-        r.n_strips = (int)g.strip_struct.size();
-        TiledArgs &a = r.targs;
-        a.structs = (const DevStruct *)(dbase + o_structs);
-        a.items = (const Item *)(dbase + g.o_items);
-        a.n_items = r.n_items;
-        a.counter = r.counter;
-        a.partials = (double *)(dbase + g.o_partials);
-        a.block = block;
-        a.variant = variant;
-        int bpc = 1;
-        HIPCHK(tiled_occupancy(fp64, g.Q, g.uniform, &bpc));
-        bpc = std::max(1, bpc);
-        const int64_t want = (r.n_items + 3) / 4;
-        r.blocks = (int)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)e->n_cu * bpc));
-        FinalizeArgs &f = r.fargs;
-        f.structs = a.structs;
-        f.csr_ptr = (const int32_t *)(dbase + g.o_csr_ptr);
-        f.csr_items = (const int32_t *)(dbase + g.o_csr_items);
-        f.partials = a.partials;
-        f.strip_struct = (const int32_t *)(dbase + g.o_ss);
-        f.strip_index = (const int32_t *)(dbase + g.o_si);
-        f.stage_ptr = (const int32_t *)(dbase + g.o_sp);
-        f.stage = (double *)(dbase + g.o_stage);
-        f.max_chunks = g.max_chunks;
-        f.Q = g.Q;
-        launches.push_back(r);
+    rc = make_launches(e, B, dbase, launches);
+    if (rc) {
+        cleanup();
+        return rc;
     }
 
     auto run = [&]() -> int {
         hipStream_t st = e->stream;
-        HIPCHK(hipMemcpyAsync(dbase, hbase, upload_bytes, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(dbase, hbase, B.upload_bytes, hipMemcpyHostToDevice, st));
         for (int s = 0; s < ns; ++s) {
             if (P[s].nq > 0)
                 HIPCHK(hipMemcpyAsync(models[s]->d_q(), P[s].q.data(), P[s].nq * 8,
                                       hipMemcpyHostToDevice, st));
         }
         HIPCHK(hipEventRecord(e->ev0, st));
-        if (plain) {
-            for (int s = 0; s < ns; ++s) {
-                if (P[s].nq == 0) continue;
-                PlainArgs a;
-                a.sd = (const DevStruct *)(dbase + o_structs) + s;
-                a.tiles = (const int4 *)(dbase + o_tiles[s]);
-                a.n_tiles = (int)P[s].tiles.size();
-                a.partials = (double *)(dbase + o_pp[s]);
-                HIPCHK(launch_plain(a, P[s].nq, st));
-            }
-        } else {
-            for (auto &r : launches) HIPCHK(run_launch(r, st, nullptr, nullptr));
-        }
+        HIPCHK(enqueue_kernels(B, P, dbase, launches, st));
         HIPCHK(hipEventRecord(e->ev1, st));
         HIPCHK(hipStreamSynchronize(st));
         return PRIPPS_OK;
     };
     e->build_serial++;
-    e->last_plain = plain;
+    e->last_plain = B.plain;
     e->last_launches = launches;
     rc = run();
     if (rc) {
@@ -729,8 +842,6 @@ static int build_models(pripps_engine *e, int32_t ns, const pripps_debye_input *
     (void)hipEventElapsedTime(&ms, e->ev0, e->ev1);
     const double total_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
-    int64_t n_items = 0;
-    for (auto &g : groups) n_items += (int64_t)g.items.size();
     for (int s = 0; s < ns; ++s) {
         e->refs.fetch_add(1);
         models[s]->build_serial = e->build_serial;
@@ -738,10 +849,10 @@ static int build_models(pripps_engine *e, int32_t ns, const pripps_debye_input *
         S.kernel_ms = ms;
         S.total_ms = total_ms;
         S.pair_q_terms = P[s].pair_q_terms;
-        S.q_per_lane = q_of[s];
-        S.block = block;
-        S.span = span_of[s];
-        S.work_items = n_items;
+        S.q_per_lane = B.plain ? 0 : B.sp[s].Q;
+        S.block = B.plain ? 0 : B.sp[s].block;
+        S.span = B.plain ? 0 : B.sp[s].span;
+        S.work_items = B.n_items;
         out[s] = models[s];
     }
     return PRIPPS_OK;
@@ -862,49 +973,6 @@ extern "C" int pripps_debye_model_fit(const pripps_debye_model *m, const double 
     return PRIPPS_OK;
 }
 
-extern "C" int pripps_debye_model_recompute(pripps_debye_model *m, int32_t repeats,
-                                            double *pair_ms_mean, double *total_ms) {
-    if (!m) return fail(PRIPPS_ERR_USAGE, "model is NULL");
-    if (repeats < 0) return fail(PRIPPS_ERR_USAGE, "repeats must be >= 0");
-    pripps_engine *e = m->engine;
-    std::lock_guard<std::mutex> lock(e->mu);
-    if (m->build_serial == 0 || m->build_serial != e->build_serial || e->last_plain)
-        return fail(PRIPPS_ERR_USAGE,
-                    "recompute needs a tiled model from the engine's most recent build");
-    HIPCHK(hipSetDevice(e->device));
-    hipStream_t st = e->stream;
-    const size_t nl = e->last_launches.size();
-    std::vector<hipEvent_t> ev(2 * (size_t)repeats * nl + 2, nullptr);
-    for (auto &x : ev) HIPCHK(hipEventCreate(&x));
-    auto destroy = [&]() {
-        for (auto &x : ev)
-            if (x) (void)hipEventDestroy(x);
-    };
-    hipError_t err = hipEventRecord(ev[0], st);
-    size_t k = 2;
-    for (int r = 0; r < repeats && err == hipSuccess; ++r)
-        for (size_t l = 0; l < nl && err == hipSuccess; ++l, k += 2)
-            err = run_launch(e->last_launches[l], st, ev[k], ev[k + 1]);
-    if (err == hipSuccess) err = hipEventRecord(ev[1], st);
-    if (err == hipSuccess) err = hipStreamSynchronize(st);
-    if (err != hipSuccess) {
-        destroy();
-        return hip_fail(err, "pripps_debye_model_recompute");
-    }
-    double sum = 0;
-    for (size_t x = 2; x < k; x += 2) {
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, ev[x], ev[x + 1]);
-        sum += ms;
-    }
-    float tot = 0;
-    (void)hipEventElapsedTime(&tot, ev[0], ev[1]);
-    destroy();
-    if (pair_ms_mean) *pair_ms_mean = repeats ? sum / repeats : 0.0;
-    if (total_ms) *total_ms = tot;
-    return PRIPPS_OK;
-}
-
 extern "C" int pripps_debye_model_from_profiles(pripps_engine *e, const double *q, int32_t nq,
                                                 const double *profiles,
                                                 pripps_debye_model **out) {
@@ -954,6 +1022,429 @@ extern "C" int pripps_debye_model_free(pripps_debye_model *m) {
     }
     delete m;
     engine_release(e);
+    return PRIPPS_OK;
+}
+
+// Replay the engine's most recent launches `repeats` times; the caller holds
+// e->mu.  pair_ms: mean device time of the pair kernels per replay.
+static int replay_timed(pripps_engine *e, int32_t repeats, double *pair_ms_mean, double *total_ms) {
+    HIPCHK(hipSetDevice(e->device));
+    hipStream_t st = e->stream;
+    const size_t nl = e->last_launches.size();
+    std::vector<hipEvent_t> ev(2 * (size_t)repeats * nl + 2, nullptr);
+    for (auto &x : ev) HIPCHK(hipEventCreate(&x));
+    auto destroy = [&]() {
+        for (auto &x : ev)
+            if (x) (void)hipEventDestroy(x);
+    };
+    hipError_t err = hipEventRecord(ev[0], st);
+    size_t k = 2;
+    for (int r = 0; r < repeats && err == hipSuccess; ++r)
+        for (size_t l = 0; l < nl && err == hipSuccess; ++l, k += 2)
+            err = run_launch(e->last_launches[l], st, ev[k], ev[k + 1]);
+    if (err == hipSuccess) err = hipEventRecord(ev[1], st);
+    if (err == hipSuccess) err = hipStreamSynchronize(st);
+    if (err != hipSuccess) {
+        destroy();
+        return hip_fail(err, "replay");
+    }
+    double sum = 0;
+    for (size_t x = 2; x < k; x += 2) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, ev[x], ev[x + 1]);
+        sum += ms;
+    }
+    float tot = 0;
+    (void)hipEventElapsedTime(&tot, ev[0], ev[1]);
+    destroy();
+    if (pair_ms_mean) *pair_ms_mean = repeats ? sum / repeats : 0.0;
+    if (total_ms) *total_ms = tot;
+    return PRIPPS_OK;
+}
+
+extern "C" int pripps_debye_model_recompute(pripps_debye_model *m, int32_t repeats,
+                                            double *pair_ms_mean, double *total_ms) {
+    if (!m) return fail(PRIPPS_ERR_USAGE, "model is NULL");
+    if (repeats < 0) return fail(PRIPPS_ERR_USAGE, "repeats must be >= 0");
+    pripps_engine *e = m->engine;
+    std::lock_guard<std::mutex> lock(e->mu);
+    if (m->build_serial == 0 || m->build_serial != e->build_serial || e->last_plain)
+        return fail(PRIPPS_ERR_USAGE,
+                    "recompute needs a tiled model from the engine's most recent build");
+    return replay_timed(e, repeats, pair_ms_mean, total_ms);
+}
+
+// ---------------------------------------------------------------------------
+// autotuning
+// ---------------------------------------------------------------------------
+static void put_entry(pripps_engine *e, const pripps_tuning_entry &x) {
+    std::lock_guard<std::mutex> lk(e->tune_mu);
+    for (auto &y : e->tune_table)
+        if (y.precision == x.precision && y.uniform == x.uniform && y.log2_atoms == x.log2_atoms &&
+            y.log2_nq == x.log2_nq) {
+            y = x;
+            return;
+        }
+    e->tune_table.push_back(x);
+}
+
+extern "C" int pripps_engine_autotune(pripps_engine *e, const pripps_debye_input *in,
+                                      int32_t precision, int32_t repeats, pripps_tuning *best,
+                                      double *best_pair_ms) {
+    if (!e) return fail(PRIPPS_ERR_USAGE, "engine is NULL");
+    if (!in) return fail(PRIPPS_ERR_USAGE, "input is NULL");
+    if (precision != PRIPPS_FP32_COMPENSATED && precision != PRIPPS_FP64)
+        return fail(PRIPPS_ERR_USAGE, "autotune: precision must be FP32_COMPENSATED or FP64");
+    repeats = std::max(1, std::min(repeats, 100));
+    Prepared p;
+    int rc = prepare(in, p);
+    if (rc) return rc;
+    if (p.n < 2 || p.nq < 1) return fail(PRIPPS_ERR_VALIDATION, "autotune needs >= 2 atoms and >= 1 q");
+
+    struct Cand {
+        int Q, shift, block, variant;
+        bool operator==(const Cand &o) const {
+            return Q == o.Q && shift == o.shift && block == o.block && variant == o.variant;
+        }
+    };
+    std::vector<std::pair<Cand, double>> seen;
+    auto absolute = [&](const Cand &c) {
+        pripps_tuning t{};
+        t.q_per_lane = c.Q;
+        t.block = c.block;
+        t.span = shifted_span(default_span(p, c.Q, e->n_cu), c.shift);
+        t.variant = c.variant;
+        return t;
+    };
+    auto eval = [&](const Cand &c, double &ms) -> int {
+        for (auto &x : seen)
+            if (x.first == c) {
+                ms = x.second;
+                return PRIPPS_OK;
+            }
+        const pripps_tuning t = absolute(c);
+        pripps_debye_model *m = nullptr;
+        int r = build_models(e, 1, in, precision, &t, &m);
+        if (r) return r;
+        {
+            std::lock_guard<std::mutex> lock(e->mu);
+            r = replay_timed(e, repeats, &ms, nullptr);
+        }
+        pripps_debye_model_free(m);
+        if (r) return r;
+        seen.push_back({c, ms});
+        return PRIPPS_OK;
+    };
+
+    Cand cur{default_q(precision, p.uniform, p.nq), 0, 64, 3};
+    double cur_ms = 0;
+    rc = eval(cur, cur_ms);
+    if (rc) return rc;
+    auto try_all = [&](std::vector<Cand> cands) -> int {
+        for (const Cand &c : cands) {
+            double ms = 0;
+            int r = eval(c, ms);
+            if (r) return r;
+            if (ms < cur_ms) {
+                cur = c;
+                cur_ms = ms;
+            }
+        }
+        return PRIPPS_OK;
+    };
+    // 1. q points per lane
+    {
+        std::vector<Cand> c;
+        const int qs[3] = {32, 64, 128};
+        for (int Q : qs)
+            if (valid_q(precision, p.uniform, Q) == Q) c.push_back({Q, 0, cur.block, 3});
+        if ((rc = try_all(c))) return rc;
+    }
+    // 2. j-span
+    {
+        std::vector<Cand> c;
+        for (int sh : {-2, -1, 1, 2}) c.push_back({cur.Q, sh, cur.block, cur.variant});
+        if ((rc = try_all(c))) return rc;
+    }
+    // 3. pair block
+    {
+        std::vector<Cand> c;
+        for (int b : {16, 32, 128, 256}) c.push_back({cur.Q, cur.shift, b, cur.variant});
+        if ((rc = try_all(c))) return rc;
+    }
+    // 4. inner-loop variant (FP32, uniform grids)
+    {
+        std::vector<Cand> c;
+        for (int v : {1, 4})
+            if (valid_variant(precision, p.uniform, cur.Q, v) == v) c.push_back({cur.Q, cur.shift, cur.block, v});
+        if ((rc = try_all(c))) return rc;
+    }
+    pripps_tuning_entry x{};
+    x.precision = precision;
+    x.uniform = p.uniform ? 1 : 0;
+    x.log2_atoms = log2_atoms_of(p.n);
+    x.log2_nq = log2_nq_of(p.nq);
+    x.q_per_lane = cur.Q;
+    x.block = cur.block;
+    x.span_shift = cur.shift;
+    x.variant = cur.variant;
+    x.pair_ms = cur_ms;
+    put_entry(e, x);
+    if (best) *best = absolute(cur);
+    if (best_pair_ms) *best_pair_ms = cur_ms;
+    return PRIPPS_OK;
+}
+
+extern "C" int pripps_engine_tuning_table(pripps_engine *e, pripps_tuning_entry *entries,
+                                          int32_t capacity, int32_t *count) {
+    if (!e || !count) return fail(PRIPPS_ERR_USAGE, "NULL argument");
+    if (capacity > 0 && !entries) return fail(PRIPPS_ERR_USAGE, "entries is NULL");
+    std::lock_guard<std::mutex> lk(e->tune_mu);
+    const int32_t n = (int32_t)e->tune_table.size();
+    for (int32_t k = 0; k < std::min(n, capacity); ++k) entries[k] = e->tune_table[k];
+    *count = n;
+    return PRIPPS_OK;
+}
+
+extern "C" int pripps_engine_set_tuning_entry(pripps_engine *e, const pripps_tuning_entry *x) {
+    if (!e || !x) return fail(PRIPPS_ERR_USAGE, "NULL argument");
+    if (x->precision != PRIPPS_FP32_COMPENSATED && x->precision != PRIPPS_FP64)
+        return fail(PRIPPS_ERR_VALIDATION, "tuning entry: precision must be FP32_COMPENSATED or FP64");
+    if (x->log2_atoms < 0 || x->log2_atoms > 31 || x->log2_nq < 0 || x->log2_nq > 30 ||
+        x->span_shift < -8 || x->span_shift > 8 || x->block < 0 || x->block > 4096)
+        return fail(PRIPPS_ERR_VALIDATION, "tuning entry out of range");
+    if (valid_q(x->precision, x->uniform != 0, x->q_per_lane) != x->q_per_lane)
+        return fail(PRIPPS_ERR_VALIDATION,
+                    "tuning entry: q_per_lane " + std::to_string(x->q_per_lane) +
+                        " is not available for this precision and grid");
+    pripps_tuning_entry y = *x;
+    y.uniform = x->uniform ? 1 : 0;
+    put_entry(e, y);
+    return PRIPPS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// trajectory frames
+// ---------------------------------------------------------------------------
+extern "C" int pripps_debye_frames_average(pripps_engine *e, const pripps_debye_input *topo,
+                                           int32_t n_frames, const double *coords,
+                                           const double *w, int32_t stride,
+                                           int32_t precision, const pripps_tuning *tune,
+                                           int32_t max_frames_per_launch,
+                                           pripps_debye_model **out) {
+    const auto t_start = std::chrono::steady_clock::now();
+    if (!out) return fail(PRIPPS_ERR_USAGE, "out is NULL");
+    *out = nullptr;
+    if (!e || !topo) return fail(PRIPPS_ERR_USAGE, "engine and topology must be non-NULL");
+    if (precision != PRIPPS_FP32_COMPENSATED && precision != PRIPPS_FP64)
+        return fail(PRIPPS_ERR_USAGE, "frames: precision must be FP32_COMPENSATED or FP64");
+    if (n_frames < 0) return fail(PRIPPS_ERR_VALIDATION, "n_frames must be >= 0");
+    if (stride < 1) return fail(PRIPPS_ERR_VALIDATION, "stride must be >= 1");
+    if (max_frames_per_launch < 0) return fail(PRIPPS_ERR_VALIDATION, "max_frames_per_launch must be >= 0");
+    const int64_t n = topo->n_atoms;
+    if (n_frames > 0 && n > 0 && !coords) return fail(PRIPPS_ERR_USAGE, "coords is NULL");
+    // weights: the file's total must be usable (read_weights), and so must
+    // the total of the frames the stride keeps (trajectory_average)
+    if (w) {
+        double sum = 0;
+        for (int32_t f = 0; f < n_frames; ++f) sum += w[f];
+        if (!std::isnormal(std::fabs(sum)))
+            return fail(PRIPPS_ERR_VALIDATION, "sum of weights is zero or infinite");
+    }
+    std::vector<int32_t> used;
+    double wsum = 0;
+    for (int32_t f = 0; f < n_frames; f += stride) {
+        used.push_back(f);
+        wsum += w ? w[f] : 1.0;
+    }
+    if (!std::isnormal(std::fabs(wsum))) {
+        char buf[64];
+        std::snprintf(buf, sizeof buf, "%g", wsum);
+        return fail(PRIPPS_ERR_VALIDATION, "weights of the " + std::to_string(used.size()) +
+                                               " averaged frames sum to " + buf +
+                                               "; cannot normalize");
+    }
+    const int32_t nq = topo->nq;
+    const int M = (int)used.size();
+
+    // per-frame host preparation (SoA coordinates of frame f)
+    std::vector<double> fx((size_t)n), fy((size_t)n), fz((size_t)n);
+    auto prep_frame = [&](int32_t f, Prepared &P) -> int {
+        const double *c = coords + (size_t)f * (size_t)n * 3;
+        for (int64_t i = 0; i < n; ++i) {
+            fx[i] = c[3 * i];
+            fy[i] = c[3 * i + 1];
+            fz[i] = c[3 * i + 2];
+        }
+        pripps_debye_input in = *topo;
+        in.x = fx.data();
+        in.y = fy.data();
+        in.z = fz.data();
+        int rc = prepare(&in, P);
+        if (rc) g_last_error = "frame " + std::to_string(f) + ": " + g_last_error;
+        return rc;
+    };
+
+    // chunk size: frames per persistent launch, bounded by a 4 GiB workspace
+    Prepared p0;
+    int rc = prep_frame(used[0], p0);
+    if (rc) return rc;
+    if (nq == 0 || p0.n == 0) return fail(PRIPPS_ERR_VALIDATION, "frames need atoms and q points");
+    int C = M;
+    {
+        BuildPlan B1;
+        std::vector<Prepared> one(1, p0);
+        plan_build(e, one, precision, tune, (size_t)nq * 8 + 8, B1);
+        const size_t per_frame = B1.total_bytes + (size_t)kProfiles * nq * 8;
+        const size_t cap = (size_t)4 << 30;
+        C = (int)std::max<size_t>(1, std::min<size_t>((size_t)M, cap / std::max<size_t>(1, per_frame)));
+    }
+    if (max_frames_per_launch > 0) C = std::min(C, (int)max_frames_per_launch);
+
+    std::lock_guard<std::mutex> lock(e->mu);
+    HIPCHK(hipSetDevice(e->device));
+    hipStream_t st = e->stream;
+    // the arena is overwritten: nothing of an earlier build stays replayable
+    e->build_serial++;
+    e->last_launches.clear();
+
+    auto *model = new pripps_debye_model();
+    model->engine = e;
+    model->nq = nq;
+    model->q = p0.q;
+    double *work = nullptr;  // [C][6 nq] frame profiles | [6 nq] accumulator
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    auto cleanup = [&]() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (work) (void)hipFree(work);
+        if (model) {
+            if (model->dev) (void)hipFree(model->dev);
+            delete model;
+        }
+    };
+#define FRAMES_CHK(expr)                     \
+    do {                                     \
+        hipError_t e_ = (expr);              \
+        if (e_ != hipSuccess) {              \
+            cleanup();                       \
+            return hip_fail(e_, #expr);      \
+        }                                    \
+    } while (0)
+    const size_t prof_n = (size_t)kProfiles * nq;
+    FRAMES_CHK(hipMalloc((void **)&model->dev, (size_t)11 * nq * sizeof(double)));
+    FRAMES_CHK(hipMalloc((void **)&work, ((size_t)C + 1) * prof_n * sizeof(double)));
+    double *acc = work + (size_t)C * prof_n;
+    FRAMES_CHK(launch_fill_double(acc, (int64_t)prof_n, 0.0, st));
+
+    double kernel_ms = 0;
+    int64_t items_per_chunk = 0;
+    int graph_launches = 0;
+    StructPlan sp0;
+    std::vector<Prepared> P;
+    for (int first = 0; first < M; first += C) {
+        const int cnt = std::min(C, M - first);
+        P.assign(cnt, Prepared{});
+        for (int c = 0; c < cnt; ++c) {
+            if (first + c == 0) {
+                P[0] = p0;
+                continue;
+            }
+            rc = prep_frame(used[first + c], P[c]);
+            if (rc) {
+                cleanup();
+                return rc;
+            }
+            if (P[c].n != p0.n || P[c].nq != p0.nq) {
+                cleanup();
+                return fail(PRIPPS_ERR_VALIDATION, "frame topology changed");
+            }
+        }
+        BuildPlan B;
+        plan_build(e, P, precision, tune, (size_t)nq * 8 + (size_t)cnt * 8, B);
+        if (first == 0) {
+            // size the arena and staging once for the largest (first) chunk, so
+            // the captured graph's pointers stay valid for every replay
+            rc = ensure_arena(e, B.total_bytes);
+            if (!rc) rc = ensure_staging(e, B.upload_bytes);
+            if (rc) {
+                cleanup();
+                return rc;
+            }
+            sp0 = B.sp[0];
+            items_per_chunk = B.n_items;
+        }
+        char *dbase = (char *)e->arena;
+        char *hbase = (char *)e->staging;
+        const double *d_q = (const double *)(dbase + B.o_user);
+        const double *d_w = d_q + nq;
+        std::vector<const double *> dq(cnt, d_q);
+        std::vector<double *> dprof(cnt);
+        for (int c = 0; c < cnt; ++c) dprof[c] = work + (size_t)c * prof_n;
+        stage_build(B, P, hbase, dbase, dq, dprof);
+        std::memcpy(hbase + B.o_user, p0.q.data(), (size_t)nq * 8);
+        double *hw = (double *)(hbase + B.o_user) + nq;
+        for (int c = 0; c < cnt; ++c) hw[c] = w ? w[used[first + c]] : 1.0;
+        std::vector<LaunchRecord> launches;
+        rc = make_launches(e, B, dbase, launches);
+        if (rc) {
+            cleanup();
+            return rc;
+        }
+        auto enqueue_all = [&]() -> hipError_t {
+            hipError_t err = hipMemcpyAsync(dbase, hbase, B.upload_bytes, hipMemcpyHostToDevice, st);
+            if (err == hipSuccess) err = enqueue_kernels(B, P, dbase, launches, st);
+            if (err == hipSuccess) err = launch_frames_accumulate(work, cnt, d_w, nq, acc, st);
+            return err;
+        };
+        const bool full = cnt == C;
+        if (full && !gexec) {
+            // capture the submission of a full chunk once; later full chunks
+            // replay it with their own frames in the same staging layout
+            FRAMES_CHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            hipError_t cerr = enqueue_all();
+            hipError_t eerr = hipStreamEndCapture(st, &graph);
+            if (cerr != hipSuccess || eerr != hipSuccess) {
+                cleanup();
+                return hip_fail(cerr != hipSuccess ? cerr : eerr, "frames: graph capture");
+            }
+            FRAMES_CHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        }
+        FRAMES_CHK(hipEventRecord(e->ev0, st));
+        if (full) {
+            FRAMES_CHK(hipGraphLaunch(gexec, st));
+            ++graph_launches;
+        } else {
+            FRAMES_CHK(enqueue_all());
+        }
+        FRAMES_CHK(hipEventRecord(e->ev1, st));
+        FRAMES_CHK(hipStreamSynchronize(st));  // staging is rewritten next chunk
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e->ev0, e->ev1);
+        kernel_ms += ms;
+    }
+    FRAMES_CHK(launch_frames_normalize(acc, nq, wsum, model->d_prof(), st));
+    FRAMES_CHK(hipMemcpyAsync(model->d_q(), p0.q.data(), (size_t)nq * 8, hipMemcpyHostToDevice, st));
+    FRAMES_CHK(hipStreamSynchronize(st));
+#undef FRAMES_CHK
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipFree(work);
+    e->refs.fetch_add(1);
+    pripps_build_stats &S = model->stats;
+    S.kernel_ms = kernel_ms;
+    S.total_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    S.pair_q_terms = p0.pair_q_terms * (double)M;
+    S.q_per_lane = sp0.Q;
+    S.block = sp0.block;
+    S.span = sp0.span;
+    S.work_items = items_per_chunk;
+    S.frames = M;
+    S.frames_per_launch = C;
+    S.graph_launches = graph_launches;
+    *out = model;
     return PRIPPS_OK;
 }
 

@@ -8,7 +8,7 @@ namespace pripps {
 
 // Maximum number of distinct atom kinds in one structure; classes are
 // (kind, hydrated) pairs, so there are at most 2 * kMaxKinds of them.
-constexpr int kMaxKinds = 16;
+constexpr int kMaxKinds = 64;
 constexpr int kMaxClasses = 2 * kMaxKinds;
 constexpr int kProfiles = 6;  // DebyeTerms order: AA EE HH AE AH EH
 constexpr int kFinChunk = 32; // item partials summed per block in the first reduction stage
@@ -37,8 +37,8 @@ struct DevStruct {
     int32_t small_h;        // dq r_max / 4 pi <= 1/8 revolution for every pair
     int32_t n_classes;
     int32_t n_strips;       // ceil(nq / Q)
-    int64_t item_begin;     // first item of this structure (items are grouped
-    int64_t csr_begin;      //   per structure); CSR of items per strip
+    int32_t block;          // j atoms summed in FP32 before the FP64 carry
+    int32_t pad_;
     int32_t cls_start[kMaxClasses + 1];
     int8_t cls_kind[kMaxClasses];
     int8_t cls_wet[kMaxClasses];
@@ -63,7 +63,6 @@ struct TiledArgs {
     int64_t n_items;
     int *counter;      // work-queue head, zeroed before launch
     double *partials;  // [n_items][6][Q]
-    int block;         // pairs summed in FP32 before the FP64 carry
     int variant;       // FP32 uniform inner-loop variant (see debye_tiled_kernel)
 };
 
@@ -96,5 +95,12 @@ hipError_t launch_plain(const PlainArgs &a, int nq, hipStream_t st);
 hipError_t launch_fill_int(int *p, int n, int v, hipStream_t st);
 hipError_t launch_combine(const double *prof, int nq, double vs, double cd, double *out,
                           hipStream_t st);
+hipError_t launch_fill_double(double *p, int64_t n, double v, hipStream_t st);
+// acc[6 nq] += sum_c w[c] prof[c][6 nq], c = 0..n_slots-1 in order
+hipError_t launch_frames_accumulate(const double *prof, int n_slots, const double *w, int nq,
+                                    double *acc, hipStream_t st);
+// out[6 nq] = acc[6 nq] / wsum
+hipError_t launch_frames_normalize(const double *acc, int nq, double wsum, double *out,
+                                   hipStream_t st);
 
 }  // namespace pripps

@@ -53,6 +53,54 @@ class Engine:
         N.check(N.load().pripps_debye_model_build_batch(self._h, len(inputs), arr, prec, tp, outs))
         return [NativeModel(ctypes.c_void_p(outs[i]), self) for i in range(len(inputs))]
 
+    def autotune(self, inp: N.PreparedInput, precision="fp32", repeats: int = 3):
+        """Time candidate plans on ``inp`` and record the fastest in this
+        engine's tuning table (used by later builds without explicit tuning of
+        the same size class).  Returns (tuning, pair-kernel ms)."""
+        prec = N.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        best = N.Tuning()
+        ms = ctypes.c_double()
+        N.check(N.load().pripps_engine_autotune(self._h, ctypes.byref(inp.c), prec, int(repeats),
+                                                ctypes.byref(best), ctypes.byref(ms)))
+        return best, ms.value
+
+    def tuning_table(self) -> List[dict]:
+        """The engine's tuning table as a list of dicts (JSON-serialisable)."""
+        L = N.load()
+        count = ctypes.c_int32()
+        N.check(L.pripps_engine_tuning_table(self._h, None, 0, ctypes.byref(count)))
+        arr = (N.TuningEntry * max(1, count.value))()
+        N.check(L.pripps_engine_tuning_table(self._h, arr, count.value, ctypes.byref(count)))
+        return [{f: getattr(arr[k], f) for f, _ in N.TuningEntry._fields_}
+                for k in range(count.value)]
+
+    def set_tuning_entry(self, entry: dict) -> None:
+        e = N.TuningEntry(*[entry[f] for f, _ in N.TuningEntry._fields_])
+        N.check(N.load().pripps_engine_set_tuning_entry(self._h, ctypes.byref(e)))
+
+    def frames_average(self, topology: N.PreparedInput, coords, weights=None, stride: int = 1,
+                       precision="fp32", tuning: Optional[N.Tuning] = None,
+                       max_frames_per_launch: int = 0) -> "NativeModel":
+        """Weighted frame average of the six profiles; ``coords`` is
+        [n_frames][n_atoms][3] in Angstrom."""
+        prec = N.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        c = np.ascontiguousarray(coords, dtype=np.float64)
+        n_atoms = len(topology.x)
+        if c.ndim != 3 or c.shape[1:] != (n_atoms, 3):
+            from .errors import ValidationError
+            raise ValidationError(f"frames must be [n_frames][{n_atoms}][3], got {list(c.shape)}")
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+        if w is not None and len(w) != c.shape[0]:
+            from .errors import ValidationError
+            raise ValidationError(f"number of weights ({len(w)}) does not match number of "
+                                  f"frames in trajectory ({c.shape[0]})")
+        h = ctypes.c_void_p()
+        tp = ctypes.byref(tuning) if tuning is not None else None
+        N.check(N.load().pripps_debye_frames_average(self._h, ctypes.byref(topology.c), c.shape[0],
+                                                     N.dptr(c), N.dptr(w), int(stride), prec, tp,
+                                                     int(max_frames_per_launch), ctypes.byref(h)))
+        return NativeModel(h, self)
+
     def from_profiles(self, q, profiles) -> "NativeModel":
         q = np.ascontiguousarray(q, dtype=np.float64)
         p = np.ascontiguousarray(profiles, dtype=np.float64).reshape(6, len(q))

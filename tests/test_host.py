@@ -190,3 +190,37 @@ def test_debye_model_requires_library_or_raises(tmp_path, monkeypatch):
     monkeypatch.setattr(_native, "LIB_PATH", str(tmp_path / "nope.so"))
     with pytest.raises(ImportError, match="no CPU fallback"):
         _native.load()
+
+
+def test_read_weights_semantics(tmp_path):
+    """Weights files as the reference's read_weights (src/trajectory.rs:117-135):
+    one number per line, no normalisation, finite non-zero total."""
+    p = tmp_path / "w.txt"
+    p.write_text("1.0\n0\n2.5e-1\n")
+    assert pripps_amd.read_weights(p).tolist() == [1.0, 0.0, 0.25]
+    p.write_text("1\r\n-1e0\r\n3\r\n")  # CRLF, signed weights
+    assert pripps_amd.read_weights(p).tolist() == [1.0, -1.0, 3.0]
+    p.write_text("1.0\n 2.0\n")  # the reference's parser does not trim
+    with pytest.raises(E.ValidationError, match="weight parse error"):
+        pripps_amd.read_weights(p)
+    p.write_text("1.0\n\n2.0\n")
+    with pytest.raises(E.ValidationError, match="weight parse error"):
+        pripps_amd.read_weights(p)
+    p.write_text("1.0\n-1.0\n")
+    with pytest.raises(E.ValidationError, match="zero or infinite"):
+        pripps_amd.read_weights(p)
+    p.write_text("inf\n1\n")
+    with pytest.raises(E.ValidationError, match="zero or infinite"):
+        pripps_amd.read_weights(p)
+    with pytest.raises(E.FileError):
+        pripps_amd.read_weights(tmp_path / "missing.txt")
+
+
+def test_tuning_entry_and_stats_layout():
+    """The ctypes mirrors of pripps_tuning_entry / pripps_build_stats have the
+    C layout (sizes and offsets of an LP64 compile of the header)."""
+    from pripps_amd import _native as N
+    assert ctypes.sizeof(N.TuningEntry) == 8 * 4 + 8
+    assert N.TuningEntry.pair_ms.offset == 32
+    assert ctypes.sizeof(N.BuildStats) == 3 * 8 + 3 * 4 + 4 + 8 + 4 * 4
+    assert N.BuildStats.work_items.offset == 40 and N.BuildStats.frames.offset == 48
