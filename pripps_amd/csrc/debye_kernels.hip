@@ -15,9 +15,9 @@
 //   sin(a + kd) - sin(a + (k-1)d) written in Reinsch's difference form: the
 //   constant lambda keeps its full relative accuracy for small d, where the
 //   plain 2 cos(d) coefficient would lose it, so the FP32 phase does not drift.
-//   Every strip is seeded afresh from hardware sin/cos, so the recurrence is
-//   refreshed every Q steps.  On non-uniform grids each term takes a direct
-//   hardware sin instead.
+//   Every strip is seeded afresh (polynomial sincos, residual-corrected r and
+//   1/r; see seed_f32), so the recurrence is refreshed every Q steps.  On
+//   non-uniform grids each term takes a direct hardware sin instead.
 //
 // Accumulation
 //   Lanes sum at most `block` pairs in FP32 (acc0/accw), then a cross-lane
@@ -226,15 +226,18 @@ __device__ __forceinline__ void sqrt_and_rsqrt(float r2, float &r, float &rinv) 
     rinv = __builtin_fmaf(y, e, y);
 }
 
-// FP32 seed: q0rev = q_s / 2pi split as q0rev + q0lo (revolutions per A),
-// hrev = dq / 4pi.  small_h: the host guarantees hrev * r <= 1/8 for every pair.
+// FP32 seed: q0rev = q_s / 2pi split as q0rev + q0lo and hrev = dq / 4pi split
+// as hrev + hlo (revolutions per A).  The lo parts remove the rounding of the
+// constants, which would otherwise shift q coherently for every pair -- an
+// error that deep minima of I(q), where |dlnI/dlnq| reaches ~100, amplify.
+// small_h: the host guarantees hrev * r <= 1/8 for every pair.
 __device__ __forceinline__ Seed32 seed_f32(float r2, float q0rev, float q0lo, float hrev,
-                                           float take, bool small_h) {
+                                           float hlo, float take, bool small_h) {
     r2 = fmaxf(r2, 1e-12f);  // coincident atoms: the r -> 0 limit s_k -> q_k
     float r, rinv;
     sqrt_and_rsqrt(r2, r, rinv);
     const float arev = __builtin_fmaf(q0rev, r, q0lo * r);
-    const float hr = hrev * r;
+    const float hr = __builtin_fmaf(hrev, r, hlo * r);
     float sa, ca, sh, chm1;
     sincos_rev(arev, sa, ca);
     if (small_h) {
@@ -305,7 +308,7 @@ __device__ __forceinline__ AtomPair32 load_pair(const PRIPPS_CONST Atom<float> *
 template <int Q, bool WET, bool SMALL_H, bool INTERLEAVE>
 __device__ __forceinline__ void pair_block_f32x2(const Atom<float> *__restrict__ atoms, int jb,
                                                  int je, const Atom<float> &ai, int iidx,
-                                                 float q0p, float q0lo, float hp,
+                                                 float q0p, float q0lo, float hp, float hlo,
                                                  float (&acc0)[Q], float (&accw)[Q]) {
     const PRIPPS_CONST Atom<float> *ca = as_const(atoms);
     int j = jb;
@@ -319,8 +322,8 @@ __device__ __forceinline__ void pair_block_f32x2(const Atom<float> *__restrict__
         const float bx = ai.x - b.x, by = ai.y - b.y, bz = ai.z - b.z;
         const float ta = j > iidx ? 1.0f : 0.0f;
         const float tb = j + 1 > iidx ? 1.0f : 0.0f;
-        const Seed32 za = seed_f32(ax * ax + ay * ay + az * az, q0p, q0lo, hp, ta, SMALL_H);
-        const Seed32 zb = seed_f32(bx * bx + by * by + bz * bz, q0p, q0lo, hp, tb, SMALL_H);
+        const Seed32 za = seed_f32(ax * ax + ay * ay + az * az, q0p, q0lo, hp, hlo, ta, SMALL_H);
+        const Seed32 zb = seed_f32(bx * bx + by * by + bz * bz, q0p, q0lo, hp, hlo, tb, SMALL_H);
         float s1 = za.s, d1 = za.d, s2 = zb.s, d2 = zb.d;
         const float l1 = za.lam, l2 = zb.lam;
         if constexpr (INTERLEAVE) {
@@ -357,7 +360,7 @@ __device__ __forceinline__ void pair_block_f32x2(const Atom<float> *__restrict__
         const Atom<float> a = load_atom(ca, j);
         const float ax = ai.x - a.x, ay = ai.y - a.y, az = ai.z - a.z;
         const float ta = j > iidx ? 1.0f : 0.0f;
-        const Seed32 za = seed_f32(ax * ax + ay * ay + az * az, q0p, q0lo, hp, ta, SMALL_H);
+        const Seed32 za = seed_f32(ax * ax + ay * ay + az * az, q0p, q0lo, hp, hlo, ta, SMALL_H);
         float s1 = za.s, d1 = za.d;
         const float l1 = za.lam;
 #pragma unroll
@@ -373,8 +376,9 @@ __device__ __forceinline__ void pair_block_f32x2(const Atom<float> *__restrict__
 template <typename T, int Q, bool UNIFORM, bool WET, bool MASK, int SMALL_H>
 __device__ __forceinline__ void pair_block(const Atom<T> *__restrict__ atoms, int jb, int je,
                                            const Atom<T> &ai, int iidx, T lane_take,
-                                           const T (&qv)[Q], T q0p, T q0lo, T hp,
-                                           bool small_h_rt, T (&acc0)[Q], T (&accw)[Q]) {
+                                           const T (&qv)[Q], const T (&ql)[Q], T q0p, T q0lo,
+                                           T hp, T hlo, bool small_h_rt, T (&acc0)[Q],
+                                           T (&accw)[Q]) {
     // SMALL_H: 1 / 0 fixed at compile time, -1 decided per pair from small_h_rt
     const bool small_h = SMALL_H < 0 ? small_h_rt : SMALL_H == 1;
     const PRIPPS_CONST Atom<T> *ca = as_const(atoms);
@@ -389,7 +393,7 @@ __device__ __forceinline__ void pair_block(const Atom<T> *__restrict__ atoms, in
         if constexpr (UNIFORM) {
             T s, d, lam;
             if constexpr (sizeof(T) == 4) {
-                Seed32 z = seed_f32(r2, q0p, q0lo, hp, take, small_h);
+                Seed32 z = seed_f32(r2, q0p, q0lo, hp, hlo, take, small_h);
                 s = z.s;
                 d = z.d;
                 lam = z.lam;
@@ -414,7 +418,9 @@ __device__ __forceinline__ void pair_block(const Atom<T> *__restrict__ atoms, in
                 const float amp = take * rinv;
 #pragma unroll
                 for (int k = 0; k < Q; ++k) {
-                    const float s = __builtin_amdgcn_sinf(__builtin_amdgcn_fractf(qv[k] * r)) * amp;
+                    // phase (q / 2pi) r with q / 2pi split as qv + ql: see seed_f32
+                    const float x = __builtin_fmaf(qv[k], r, ql[k] * r);
+                    const float s = __builtin_amdgcn_sinf(__builtin_amdgcn_fractf(x)) * amp;
                     acc0[k] += s;
                     if constexpr (WET) accw[k] += aj.w * s;
                 }
@@ -464,26 +470,34 @@ __global__ void __launch_bounds__(256) debye_tiled_kernel(TiledArgs args) {
         const int qbase = it.strip * Q;
         const bool small_h = sd.small_h != 0;
         // strip constants
-        T qv[Q];
-        T q0p = T(0), q0lo = T(0), hp = T(0);
+        T qv[Q], ql[Q];
+        T q0p = T(0), q0lo = T(0), hp = T(0), hlo = T(0);
         if constexpr (UNIFORM) {
             const double qs = sd.q0 + double(qbase) * sd.dq;
             if constexpr (sizeof(T) == 4) {
                 q0p = float(qs * kInv2Pi);
                 q0lo = float(qs * kInv2Pi - double(q0p));
-                hp = float(sd.dq * (0.5 * kInv2Pi));
+                const double hd = sd.dq * (0.5 * kInv2Pi);
+                hp = float(hd);
+                hlo = float(hd - double(hp));
             } else {
                 q0p = qs;
                 hp = 0.5 * sd.dq;
             }
 #pragma unroll
-            for (int k = 0; k < Q; ++k) qv[k] = T(0);
+            for (int k = 0; k < Q; ++k) qv[k] = ql[k] = T(0);
         } else {
 #pragma unroll
             for (int k = 0; k < Q; ++k) {
                 const int m = min(qbase + k, nq - 1);
-                if constexpr (sizeof(T) == 4) qv[k] = float(sd.q[m] * kInv2Pi);
-                else qv[k] = sd.q[m];
+                if constexpr (sizeof(T) == 4) {
+                    const double qd = sd.q[m] * kInv2Pi;
+                    qv[k] = float(qd);
+                    ql[k] = float(qd - double(qv[k]));
+                } else {
+                    qv[k] = sd.q[m];
+                    ql[k] = T(0);
+                }
             }
         }
 
@@ -514,14 +528,14 @@ __global__ void __launch_bounds__(256) debye_tiled_kernel(TiledArgs args) {
 
             if constexpr (UNIFORM && sizeof(T) == 4 && (VARIANT == 1 || VARIANT == 2)) {
                 if (wet) {
-                    if (small_h) pair_block_f32x2<Q, true, true, INTERLEAVE>(atoms, j, jend, ai, iidx, q0p, q0lo, hp, acc0, accw);
-                    else pair_block_f32x2<Q, true, false, INTERLEAVE>(atoms, j, jend, ai, iidx, q0p, q0lo, hp, acc0, accw);
+                    if (small_h) pair_block_f32x2<Q, true, true, INTERLEAVE>(atoms, j, jend, ai, iidx, q0p, q0lo, hp, hlo, acc0, accw);
+                    else pair_block_f32x2<Q, true, false, INTERLEAVE>(atoms, j, jend, ai, iidx, q0p, q0lo, hp, hlo, acc0, accw);
                 } else {
-                    if (small_h) pair_block_f32x2<Q, false, true, INTERLEAVE>(atoms, j, jend, ai, iidx, q0p, q0lo, hp, acc0, accw);
-                    else pair_block_f32x2<Q, false, false, INTERLEAVE>(atoms, j, jend, ai, iidx, q0p, q0lo, hp, acc0, accw);
+                    if (small_h) pair_block_f32x2<Q, false, true, INTERLEAVE>(atoms, j, jend, ai, iidx, q0p, q0lo, hp, hlo, acc0, accw);
+                    else pair_block_f32x2<Q, false, false, INTERLEAVE>(atoms, j, jend, ai, iidx, q0p, q0lo, hp, hlo, acc0, accw);
                 }
             } else if constexpr (UNIFORM && sizeof(T) == 4 && VARIANT == 4) {
-#define PRIPPS_PB(W, M, S) pair_block<T, Q, UNIFORM, W, M, S>(atoms, j, jend, ai, iidx, lane_take, qv, q0p, q0lo, hp, small_h, acc0, accw)
+#define PRIPPS_PB(W, M, S) pair_block<T, Q, UNIFORM, W, M, S>(atoms, j, jend, ai, iidx, lane_take, qv, ql, q0p, q0lo, hp, hlo, small_h, acc0, accw)
                 if (wet) {
                     if (mask) { if (small_h) PRIPPS_PB(true, true, 1); else PRIPPS_PB(true, true, 0); }
                     else { if (small_h) PRIPPS_PB(true, false, 1); else PRIPPS_PB(true, false, 0); }

@@ -404,7 +404,7 @@ int default_q(int precision, bool uniform, int nq) {
     return nq <= 32 ? 32 : 64;
 }
 
-// j atoms per work item: about 8 items per wave slot of the GPU when the
+// j atoms per work item: about 16 items per wave slot of the GPU when the
 // structure runs alone, 256..8192, a multiple of 64.  It depends on the
 // structure only, so a structure is cut into the same items alone or in any
 // batch, and its profiles are bitwise the same either way.
@@ -412,9 +412,12 @@ int default_span(const Prepared &p, int Q, int n_cu) {
     const double strips = (double)((p.nq + Q - 1) / Q);
     const double work = (double)p.tiles.size() * 0.5 * p.n * strips;
     const double slots = std::max(1, n_cu) * 8.0;
-    const int span = (int)(work / (8.0 * slots));
+    const int span = (int)(work / (16.0 * slots));
     return std::max(256, std::min(8192, (span + 63) / 64 * 64));
 }
+
+// j atoms summed in FP32 per lane before the FP64 carry
+constexpr int kDefaultBlock = 256;
 
 int shifted_span(int base, int shift) {
     double s = std::ldexp((double)base, shift);
@@ -490,7 +493,8 @@ static StructPlan resolve_plan(pripps_engine *e, const pripps_tuning *tune, int 
     if (expl && tune->span > 0) sp.span = std::max(64, tune->span);
     else if (have) sp.span = shifted_span(base, te.span_shift);
     else sp.span = base;
-    const int block = expl && tune->block > 0 ? tune->block : (have && te.block > 0 ? te.block : 64);
+    const int block =
+        expl && tune->block > 0 ? tune->block : (have && te.block > 0 ? te.block : kDefaultBlock);
     sp.block = std::max(8, std::min(block, 4096));
     sp.variant = valid_variant(precision, p.uniform, Q, expl ? tune->variant : (have ? te.variant : 3));
     return sp;
@@ -1136,7 +1140,7 @@ extern "C" int pripps_engine_autotune(pripps_engine *e, const pripps_debye_input
         return PRIPPS_OK;
     };
 
-    Cand cur{default_q(precision, p.uniform, p.nq), 0, 64, 3};
+    Cand cur{default_q(precision, p.uniform, p.nq), 0, kDefaultBlock, 3};
     double cur_ms = 0;
     rc = eval(cur, cur_ms);
     if (rc) return rc;
@@ -1169,7 +1173,7 @@ extern "C" int pripps_engine_autotune(pripps_engine *e, const pripps_debye_input
     // 3. pair block
     {
         std::vector<Cand> c;
-        for (int b : {16, 32, 128, 256}) c.push_back({cur.Q, cur.shift, b, cur.variant});
+        for (int b : {32, 64, 128, 512}) c.push_back({cur.Q, cur.shift, b, cur.variant});
         if ((rc = try_all(c))) return rc;
     }
     // 4. inner-loop variant (FP32, uniform grids)

@@ -60,10 +60,10 @@ struct Request {
 
 uint64_t estimate_bytes(const pripps_debye_input &in) {
     // model (11 nq doubles) + sorted atoms + tables + a share of the partial
-    // profile buffer (one 6 x 64 FP64 slot per 64 x 512 pair tile and strip)
+    // profile buffer (one 6 x 64 FP64 slot per 64 x 256 pair tile and strip)
     const double n = (double)in.n_atoms, nq = (double)in.nq;
     const double tiles = n / 64.0 + 16.0;
-    const double spans = n / 512.0 + 1.0;
+    const double spans = n / 256.0 + 1.0;
     const double strips = nq / 64.0 + 1.0;
     const double partial = tiles * spans * strips * 6.0 * 64.0 * 8.0 * 0.5;
     return (uint64_t)(11.0 * nq * 8.0 + n * 40.0 + 2.0 * 16.0 * nq * 8.0 + partial + 4096.0);
