@@ -177,6 +177,16 @@ int pripps_debye_model_fit(const pripps_debye_model *model, const double *q,
                            pripps_fit_result *out);
 
 /*
+ * The fitted theory curve at experimental q: the natural cubic spline of the
+ * combined total I(q; volume_scale, contrast_density) over the model's grid,
+ * evaluated at q[0..n) (constant beyond the ends) -- the i_fit column of
+ * write_fit_csv (src/fit.rs:548-560) before its scale factor.  FP64.
+ */
+int pripps_debye_model_fit_curve(const pripps_debye_model *model, double volume_scale,
+                                 double contrast_density, const double *q, int64_t n,
+                                 double *out);
+
+/*
  * A model from previously exported profiles ([6][nq], pripps_debye_model_profiles)
  * and their q grid: checkpoint / resume of a cached model without rebuilding.
  * The reference's DebyeModel is not serialisable (src/debye.rs:139-142); its

@@ -35,6 +35,7 @@ from .engine import Engine, NativeModel, default_engine
 from .errors import (DeviceError, FileError, InternalError, OutOfDeviceMemoryError,
                      PrippsError, QOutOfRangeError, UndefinedFormFactorError, UnknownAtomError,
                      UsageError, ValidationError, YamlError)
+from .formats import ExpData, read_exp_data, write_fit_csv, write_intensity_csv
 from .formfactor import FormFactorMap, read_formfactors, read_formfactors_merged
 from .solvent import excluded_name, excluded_table, hydration_name, water_table
 from .structure import Structure, resolve_ids, structure_from_file
@@ -132,6 +133,11 @@ class DebyeModel:
         r = self._native.fit(q, intensity, sigma, _fit_method(method), c1_points, c2_points,
                              c1_bounds, c2_bounds)
         return FitResult(r.volume_scale, r.contrast_density, r.chi2, r.scale)
+
+    def fit_curve(self, q, volume_scale: float, contrast_density: float) -> np.ndarray:
+        """The theory curve at experimental q as the reference's fit CSV
+        evaluates it: natural spline of I(q; vs, cd) (src/fit.rs:548-560)."""
+        return self._native.fit_curve(q, volume_scale, contrast_density)
 
     def stats(self):
         return self._native.stats()
@@ -312,7 +318,8 @@ def read_weights(path) -> np.ndarray:
 
 __all__ = [
     "Pripps", "DebyeModel", "Intensity", "FitResult", "Excluded", "Hydration", "FitMethod",
-    "Engine", "read_weights", "DEFAULT_C1_BOUNDS", "DEFAULT_C2_BOUNDS", "PrippsError", "FileError", "YamlError",
+    "Engine", "read_weights", "ExpData", "read_exp_data", "write_intensity_csv", "write_fit_csv",
+    "DEFAULT_C1_BOUNDS", "DEFAULT_C2_BOUNDS", "PrippsError", "FileError", "YamlError",
     "UnknownAtomError", "UndefinedFormFactorError", "QOutOfRangeError", "ValidationError",
     "UsageError", "InternalError", "DeviceError", "OutOfDeviceMemoryError",
 ]

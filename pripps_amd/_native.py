@@ -83,7 +83,8 @@ EXPORTS = (
     "pripps_last_error", "pripps_version", "pripps_engine_create", "pripps_engine_destroy",
     "pripps_engine_memory", "pripps_debye_model_build", "pripps_debye_model_build_batch",
     "pripps_debye_model_nq", "pripps_debye_model_q_values", "pripps_debye_model_intensity",
-    "pripps_debye_model_profiles", "pripps_debye_model_fit", "pripps_debye_model_from_profiles",
+    "pripps_debye_model_profiles", "pripps_debye_model_fit", "pripps_debye_model_fit_curve",
+    "pripps_debye_model_from_profiles",
     "pripps_debye_model_stats", "pripps_debye_model_recompute",
     "pripps_debye_model_free", "pripps_engine_autotune", "pripps_engine_tuning_table",
     "pripps_engine_set_tuning_entry", "pripps_debye_frames_average", "pripps_server_create",
@@ -124,6 +125,8 @@ def load():
         L.pripps_debye_model_fit.argtypes = [vp, _D, _D, _D, ctypes.c_int64, ctypes.c_int32,
                                              ctypes.c_int32, ctypes.c_int32, _D, _D,
                                              ctypes.POINTER(FitResultC)]
+        L.pripps_debye_model_fit_curve.argtypes = [vp, ctypes.c_double, ctypes.c_double, _D,
+                                                   ctypes.c_int64, _D]
         L.pripps_debye_model_from_profiles.argtypes = [vp, _D, ctypes.c_int32, _D,
                                                        ctypes.POINTER(vp)]
         L.pripps_debye_model_recompute.argtypes = [vp, ctypes.c_int32,

@@ -29,6 +29,8 @@ int device_fit(hipStream_t st, const double *d_prof, const double *d_q, int nq, 
                const double *inten, const double *sigma, int64_t n, int method, int c1_points,
                int c2_points, const double *c1b, const double *c2b, pripps_fit_result *out,
                std::string *err);
+int device_spline_total(hipStream_t st, const double *d_prof, const double *d_q, int nq, double vs,
+                        double cd, const double *qe, int64_t n, double *out, std::string *err);
 
 }  // namespace pripps
 
@@ -1026,6 +1028,23 @@ extern "C" int pripps_debye_model_free(pripps_debye_model *m) {
     }
     delete m;
     engine_release(e);
+    return PRIPPS_OK;
+}
+
+extern "C" int pripps_debye_model_fit_curve(const pripps_debye_model *m, double vs, double cd,
+                                            const double *q, int64_t n, double *out) {
+    if (!m) return fail(PRIPPS_ERR_USAGE, "model is NULL");
+    if (n < 0) return fail(PRIPPS_ERR_VALIDATION, "n must be >= 0");
+    if (n > 0 && (!q || !out)) return fail(PRIPPS_ERR_USAGE, "q and out must be non-NULL");
+    if (n == 0) return PRIPPS_OK;
+    if (m->nq == 0) return fail(PRIPPS_ERR_VALIDATION, "model has no q points");
+    if (!finite_all(q, (size_t)n)) return fail(PRIPPS_ERR_VALIDATION, "q must be finite");
+    pripps_engine *e = m->engine;
+    std::lock_guard<std::mutex> lock(e->mu);
+    HIPCHK(hipSetDevice(e->device));
+    std::string err;
+    int rc = device_spline_total(e->stream, m->d_prof(), m->d_q(), m->nq, vs, cd, q, n, out, &err);
+    if (rc) return fail(rc, err);
     return PRIPPS_OK;
 }
 

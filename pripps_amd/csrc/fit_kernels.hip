@@ -21,11 +21,12 @@
 
 namespace pripps {
 
-// y2 of the natural spline through (q, profile_p), one thread per profile.
+// y2 of the natural spline through (q, curve_p), one thread per curve.
 __global__ void spline_y2_kernel(const double *__restrict__ prof, const double *__restrict__ xs,
-                                 int n, double *__restrict__ y2all, double *__restrict__ uall) {
+                                 int n, int n_curves, double *__restrict__ y2all,
+                                 double *__restrict__ uall) {
     const int p = threadIdx.x;
-    if (p >= kProfiles) return;
+    if (p >= n_curves) return;
     const double *ys = prof + (size_t)p * n;
     double *y2 = y2all + (size_t)p * n;
     double *u = uall + (size_t)p * n;
@@ -44,9 +45,9 @@ __global__ void spline_y2_kernel(const double *__restrict__ prof, const double *
     for (int i = n - 2; i >= 0; --i) y2[i] = y2[i] * y2[i + 1] + u[i];
 }
 
-// M[p][e] = spline_p(qe[e]) for the six profiles.
+// M[p][e] = spline_p(qe[e]) for n_curves curves.
 __global__ void spline_map_kernel(const double *__restrict__ prof, const double *__restrict__ y2all,
-                                  const double *__restrict__ xs, int n,
+                                  const double *__restrict__ xs, int n, int n_curves,
                                   const double *__restrict__ qe, int ne, double *__restrict__ M) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= ne) return;
@@ -65,7 +66,7 @@ __global__ void spline_map_kernel(const double *__restrict__ prof, const double 
         i = lo;
         mode = i == 0 ? 0 : 2;
     }
-    for (int p = 0; p < kProfiles; ++p) {
+    for (int p = 0; p < n_curves; ++p) {
         const double *ys = prof + (size_t)p * n;
         const double *y2 = y2all + (size_t)p * n;
         double v;
@@ -288,9 +289,10 @@ int device_fit(hipStream_t st, const double *d_prof, const double *d_q, int nq, 
     ExpView v{d_M, d_exp + ne, d_exp + 2 * ne, ne};
     do {
         if ((e = hipMemcpyAsync(d_exp, host.data(), host.size() * 8, hipMemcpyHostToDevice, st)) != hipSuccess) break;
-        hipLaunchKernelGGL(spline_y2_kernel, dim3(1), dim3(64), 0, st, d_prof, d_q, nq, d_y2, d_u);
+        hipLaunchKernelGGL(spline_y2_kernel, dim3(1), dim3(64), 0, st, d_prof, d_q, nq, kProfiles,
+                           d_y2, d_u);
         hipLaunchKernelGGL(spline_map_kernel, dim3((ne + 255) / 256), dim3(256), 0, st, d_prof, d_y2,
-                           d_q, nq, (const double *)d_exp, ne, d_M);
+                           d_q, nq, kProfiles, (const double *)d_exp, ne, d_M);
         if (method == PRIPPS_FIT_GRID) {
             hipLaunchKernelGGL(fit_grid_kernel, dim3((unsigned)((grid + 255) / 256)), dim3(256), 0, st, v,
                                c1p, c2p, c1b[0], c1b[1], c2b[0], c2b[1], d_chi);
@@ -318,6 +320,45 @@ int device_fit(hipStream_t st, const double *d_prof, const double *d_q, int nq, 
     out->contrast_density = res[1];
     out->chi2 = res[2];
     out->scale = res[3];
+    return PRIPPS_OK;
+}
+
+// The theory curve at experimental q as write_fit_csv evaluates it
+// (src/fit.rs:548-560): the natural spline of the combined total I(q) over
+// the model's grid, evaluated at qe.  FP64 on the device.
+int device_spline_total(hipStream_t st, const double *d_prof, const double *d_q, int nq, double vs,
+                        double cd, const double *qe, int64_t n, double *out, std::string *err) {
+    const int ne = (int)n;
+    double *buf = nullptr;
+    const size_t doubles = (size_t)4 * nq + (size_t)2 * nq + (size_t)2 * ne + 8;
+    hipError_t e = hipMalloc((void **)&buf, doubles * sizeof(double));
+    if (e != hipSuccess) {
+        *err = std::string("hipMalloc(fit curve): ") + hipGetErrorString(e);
+        return e == hipErrorOutOfMemory ? PRIPPS_ERR_OUT_OF_MEMORY : PRIPPS_ERR_DEVICE;
+    }
+    double *d_tot = buf;             // [4][nq]: combine output, row 0 = total
+    double *d_y2 = d_tot + (size_t)4 * nq;
+    double *d_u = d_y2 + nq;
+    double *d_qe = d_u + nq;
+    double *d_out = d_qe + ne;
+    do {
+        if ((e = hipMemcpyAsync(d_qe, qe, (size_t)ne * 8, hipMemcpyHostToDevice, st)) != hipSuccess) break;
+        if ((e = launch_combine(d_prof, nq, vs, cd, d_tot, st)) != hipSuccess) break;
+        hipLaunchKernelGGL(spline_y2_kernel, dim3(1), dim3(64), 0, st, (const double *)d_tot, d_q, nq,
+                           1, d_y2, d_u);
+        hipLaunchKernelGGL(spline_map_kernel, dim3((ne + 255) / 256), dim3(256), 0, st,
+                           (const double *)d_tot, (const double *)d_y2, d_q, nq, 1,
+                           (const double *)d_qe, ne, d_out);
+        if ((e = hipGetLastError()) != hipSuccess) break;
+        if ((e = hipMemcpyAsync(out, d_out, (size_t)ne * 8, hipMemcpyDeviceToHost, st)) != hipSuccess) break;
+        e = hipStreamSynchronize(st);
+    } while (0);
+    hipError_t ef = hipFree(buf);
+    if (e == hipSuccess) e = ef;
+    if (e != hipSuccess) {
+        *err = std::string("fit curve: ") + hipGetErrorString(e);
+        return PRIPPS_ERR_DEVICE;
+    }
     return PRIPPS_OK;
 }
 

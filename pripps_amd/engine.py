@@ -177,6 +177,15 @@ class NativeModel:
                                                 ctypes.byref(res)))
         return res
 
+    def fit_curve(self, q, volume_scale: float, contrast_density: float) -> np.ndarray:
+        """Natural spline of the combined I(q) evaluated at ``q`` (device)."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        out = np.empty(len(q))
+        N.check(N.load().pripps_debye_model_fit_curve(self._h, float(volume_scale),
+                                                      float(contrast_density), N.dptr(q), len(q),
+                                                      N.dptr(out)))
+        return out
+
     def recompute(self, repeats: int):
         """Re-run the last build's kernels on resident inputs; returns
         (mean pair-kernel ms, total device ms)."""

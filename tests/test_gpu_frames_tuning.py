@@ -211,3 +211,36 @@ def test_pepsi_kat_twenty_kinds_on_gpu(engine, oracle, precision, tol):
     orc = oracle.build_model(st.pos, kind, q, p.formfactors.table(used, q), mode=oracle.COMPENSATED)
     assert np.abs(m.profiles()[0] - orc[0]).max() <= tol * np.abs(orc[0]).max()
     assert np.abs((m.profiles()[0] - orc[0]) / orc[0]).max() <= tol
+
+
+def test_fit_curve_and_fit_csv(engine, foxs, oracle, tmp_path):
+    """write_fit_csv (src/fit.rs:548-560): i_fit is scale x the natural spline
+    of the fitted total, computed on the device; checked against the oracle's
+    spline of the same combined curve."""
+    import pripps_amd
+    pos = random_cluster(600, 31)
+    ids = np.arange(600) % 4
+    m = foxs.debye_model_from_arrays(pos, ids, qmin=0.0, qmax=0.5, nq=120, excluded="foxs",
+                                     precision="fp64")
+    exp = pripps_amd.read_exp_data(os.path.join(REFDATA, "lyzexp.dat"))
+    qe = exp.q[exp.q <= 0.55]
+    vs, cd = 1.03, 0.02
+    got = m.fit_curve(qe, vs, cd)
+    total = m.intensity(vs, cd).total
+    ref = oracle.spline_eval(m.q_values, total, qe)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    e2 = pripps_amd.ExpData(qe, 2.5 * ref, np.full(len(qe), 0.5))
+    fit = m.fit(e2.q, e2.intensity, e2.sigma, method="grid", c1_points=21, c2_points=11,
+                c1_bounds=(0.93, 1.13), c2_bounds=(0.0, 0.1))
+    path = tmp_path / "fit.csv"
+    pripps_amd.write_fit_csv(path, e2, m, fit)
+    lines = path.read_text().splitlines()
+    assert lines[0] == "q,i_exp,i_fit,sigma" and len(lines) == len(qe) + 1
+    rows = np.array([[float(x) for x in ln.split(",")] for ln in lines[1:]])
+    assert np.allclose(rows[:, 0], qe, atol=6e-7, rtol=0)
+    i_fit = fit.scale * m.fit_curve(qe, fit.volume_scale, fit.contrast_density)
+    assert np.allclose(rows[:, 2], i_fit, atol=6e-7, rtol=1e-15)
+    # no hydration layer: contrast_density has no effect and the grid keeps its
+    # first (row-major) minimum, c2 = 0
+    assert np.abs(fit.volume_scale - vs) < 1e-9 and fit.contrast_density == 0.0
+    assert abs(fit.scale - 2.5) < 1e-9 and fit.chi2 < 1e-10

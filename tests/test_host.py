@@ -224,3 +224,48 @@ def test_tuning_entry_and_stats_layout():
     assert N.TuningEntry.pair_ms.offset == 32
     assert ctypes.sizeof(N.BuildStats) == 3 * 8 + 3 * 4 + 4 + 8 + 4 * 4
     assert N.BuildStats.work_items.offset == 40 and N.BuildStats.frames.offset == 48
+
+
+def test_read_exp_data_lyzexp():
+    """The reference's load_lyzexp (src/fit.rs:577-584) on its own data file."""
+    exp = pripps_amd.read_exp_data(os.path.join(REFDATA, "lyzexp.dat"))
+    assert len(exp.q) > 100
+    assert exp.q[0] > 0 and exp.intensity[0] > 0 and exp.sigma[0] > 0
+
+
+def test_read_exp_data_rules(tmp_path):
+    """ExpData::from_file's line rules (src/fit.rs:28-84)."""
+    p = tmp_path / "d.dat"
+    p.write_text("# comment\nq I sigma\n\n0.1, 2.0, 0.5\n 0.2 3.0\n0.3\t4.0 abc\nabc 1 2\n"
+                 "0.4 x\n1_0 2\n+0.5 .25 1e-1\n0.6,7\r\n0.7\n")
+    exp = pripps_amd.read_exp_data(p)
+    assert exp.q.tolist() == [0.1, 0.2, 0.3, 0.5, 0.6]
+    assert exp.intensity.tolist() == [2.0, 3.0, 4.0, 0.25, 7.0]
+    assert exp.sigma.tolist() == [0.5, 1.0, 1.0, 0.1, 1.0]
+    p.write_text("# nothing\nq I\n")
+    with pytest.raises(E.ValidationError, match="no data points"):
+        pripps_amd.read_exp_data(p)
+    with pytest.raises(E.FileError):
+        pripps_amd.read_exp_data(tmp_path / "missing.dat")
+    a = pripps_amd.ExpData.from_arrays([0.1, 0.2], [1.0, 2.0])
+    assert a.sigma.tolist() == [1.0, 1.0]
+    with pytest.raises(E.ValidationError, match="length mismatch"):
+        pripps_amd.ExpData.from_arrays([0.1, 0.2], [1.0, 2.0], [1.0])
+    with pytest.raises(E.ValidationError, match="empty"):
+        pripps_amd.ExpData.from_arrays([], [])
+
+
+def test_write_intensity_csv(tmp_path):
+    """The CLI's CSV (src/cli.rs:510-551): header, then `{:.6}` fields."""
+    q = np.array([0.0, 0.25, 0.5])
+    full = pripps_amd.Intensity(q, np.array([1.0, -0.0, float("nan")]), np.array([2.0, 3.0, 4.0]),
+                                np.array([0.0, 1e-7, 7.5e-7]), np.array([float("inf"), -1.5, 2.25]))
+    path = tmp_path / "i.csv"
+    pripps_amd.write_intensity_csv(path, full)
+    assert path.read_text() == ("q,total,atoms,excluded,hydration\n"
+                                "0.000000,1.000000,2.000000,0.000000,inf\n"
+                                "0.250000,-0.000000,3.000000,0.000000,-1.500000\n"
+                                "0.500000,NaN,4.000000,0.000001,2.250000\n")
+    bare = pripps_amd.Intensity(q[:1], np.array([12.5]))
+    pripps_amd.write_intensity_csv(path, bare)
+    assert path.read_text() == "q,total\n0.000000,12.500000\n"
