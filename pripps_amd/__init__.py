@@ -37,7 +37,9 @@ from .errors import (DeviceError, FileError, InternalError, OutOfDeviceMemoryErr
                      UsageError, ValidationError, YamlError)
 from .formats import ExpData, read_exp_data, write_fit_csv, write_intensity_csv
 from .formfactor import FormFactorMap, read_formfactors, read_formfactors_merged
-from .solvent import excluded_name, excluded_table, hydration_name, water_table
+from .solvent import (atom_radii, excluded_name, excluded_table, grid_excluded_dummies,
+                      grid_excluded_form_factor, grid_hydration_dummies, hydration_name,
+                      molecules_per_cell, water_table)
 from .structure import Structure, resolve_ids, structure_from_file
 
 __version__ = "0.1.0"
@@ -165,12 +167,18 @@ class Pripps:
 
     def prepare_input(self, pos: np.ndarray, ids: np.ndarray, q: np.ndarray, *,
                       excluded=None, hydration=None, hydration_weights=None,
-                      bulk_electron_density: float = 0.334) -> _native.PreparedInput:
+                      bulk_electron_density: float = 0.334, probe: float = 1.8,
+                      spacing: float = 3.10516, thickness: float = 3.0,
+                      excluded_spacing: float = 1.0) -> _native.PreparedInput:
         """Host tables for one structure: used kinds only (as
-        formfactor_by_index evaluates only referenced kinds)."""
+        formfactor_by_index evaluates only referenced kinds).  Grid solvent
+        models append their dummies as points of extra kinds (see
+        ``pripps_amd.solvent``): excluded-grid cells carry only the cell's
+        excluded form factor, hydration-grid cells only a water weight."""
         ex = excluded_name(excluded)
         hy = hydration_name(hydration)
         ids = np.asarray(ids, dtype=np.int64)
+        pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
         used = sorted(set(int(i) for i in ids)) or [0]
         remap = {k: r for r, k in enumerate(used)}
         kind = np.array([remap[int(i)] for i in ids], dtype=np.int32)
@@ -178,7 +186,35 @@ class Pripps:
         atom_ff = self.formfactors.table(used, q)
         excl = excluded_table(ex, self.formfactors, used, q, bulk_electron_density)
         water, w = water_table(hy, self.formfactors, q, hydration_weights, len(ids))
-        return _native.PreparedInput(pos, kind, q, atom_ff, excl, water, w)
+        if ex != "grid" and hy != "grid":
+            return _native.PreparedInput(pos, kind, q, atom_ff, excl, water, w)
+        points, kinds, weights = [pos], [kind], [w] if water is not None else None
+        tables = [atom_ff]
+        etables = [excl] if excl is not None else None
+        nq = len(q)
+        if ex == "grid":
+            radii = atom_radii(self.formfactors, ids, "ExcludedVolume::Grid")
+            E = grid_excluded_dummies(pos, radii, float(excluded_spacing))
+            points.append(E)
+            kinds.append(np.full(len(E), len(tables[0]) + len(tables) - 1, dtype=np.int32))
+            tables.append(np.zeros((1, nq)))
+            etables.append(grid_excluded_form_factor(q, bulk_electron_density,
+                                                     float(excluded_spacing))[None, :])
+            if weights is not None:
+                weights.append(np.zeros(len(E)))
+        if hy == "grid":
+            radii = atom_radii(self.formfactors, ids, "Hydration::Grid")
+            H = grid_hydration_dummies(pos, radii, float(spacing), float(thickness), float(probe))
+            points.append(H)
+            kinds.append(np.full(len(H), len(tables[0]) + len(tables) - 1, dtype=np.int32))
+            tables.append(np.zeros((1, nq)))
+            if etables is not None:
+                etables.append(np.zeros((1, nq)))
+            weights.append(np.full(len(H), molecules_per_cell(float(spacing))))
+        return _native.PreparedInput(
+            np.vstack(points), np.concatenate(kinds), q, np.vstack(tables),
+            None if etables is None else np.vstack(etables), water,
+            None if weights is None else np.concatenate(weights))
 
     def debye_model(self, structure_file, *, qmin: float = 0.0, qmax: float = 0.5, nq: int = 100,
                     excluded=None, hydration=None, probe: float = 1.8, spacing: float = 3.10516,
@@ -187,22 +223,25 @@ class Pripps:
                     hydration_weights=None, precision: str = "fp32",
                     tuning: Optional[_native.Tuning] = None) -> DebyeModel:
         """Build a cached Debye model on the GPU (reference
-        ``Pripps::debye_model``, src/lib.rs:287-303).  ``probe``, ``spacing``,
-        ``thickness`` and ``excluded_spacing`` only parameterise dummy
-        generators this engine does not offer and are accepted for signature
-        compatibility."""
+        ``Pripps::debye_model``, src/lib.rs:287-303).  ``spacing``,
+        ``thickness`` and ``probe`` parameterise ``hydration="grid"``
+        (pripps-py/src/lib.rs:731-758), ``excluded_spacing`` parameterises
+        ``excluded="grid"``."""
         st = structure_from_file(structure_file, self.formfactors.kinds())
         return self.debye_model_from_arrays(st.pos, st.ids, qmin=qmin, qmax=qmax, nq=nq,
                                             excluded=excluded, hydration=hydration,
                                             bulk_electron_density=bulk_electron_density,
                                             q_values=q_values,
                                             hydration_weights=hydration_weights,
-                                            precision=precision, tuning=tuning)
+                                            precision=precision, tuning=tuning, probe=probe,
+                                            spacing=spacing, thickness=thickness,
+                                            excluded_spacing=excluded_spacing)
 
     def debye_model_from_arrays(self, pos, ids, *, qmin=0.0, qmax=0.5, nq=100, excluded=None,
                                 hydration=None, bulk_electron_density=0.334, q_values=None,
                                 hydration_weights=None, precision="fp32",
-                                tuning=None) -> DebyeModel:
+                                tuning=None, probe=1.8, spacing=3.10516, thickness=3.0,
+                                excluded_spacing=1.0) -> DebyeModel:
         if q_values is None:
             nq = int(nq)
             if nq < 0:
@@ -212,7 +251,9 @@ class Pripps:
             q = np.ascontiguousarray(q_values, dtype=np.float64)
         inp = self.prepare_input(pos, ids, q, excluded=excluded, hydration=hydration,
                                  hydration_weights=hydration_weights,
-                                 bulk_electron_density=bulk_electron_density)
+                                 bulk_electron_density=bulk_electron_density, probe=probe,
+                                 spacing=spacing, thickness=thickness,
+                                 excluded_spacing=excluded_spacing)
         native = self.engine.build(inp, precision, tuning)
         return DebyeModel(native, q)
 

@@ -231,30 +231,49 @@ __device__ __forceinline__ void sqrt_and_rsqrt(float r2, float &r, float &rinv) 
 // constants, which would otherwise shift q coherently for every pair -- an
 // error that deep minima of I(q), where |dlnI/dlnq| reaches ~100, amplify.
 // small_h: the host guarantees hrev * r <= 1/8 for every pair.
-__device__ __forceinline__ Seed32 seed_f32(float r2, float q0rev, float q0lo, float hrev,
-                                           float hlo, float take, bool small_h) {
+// The strip-independent part of a pair's seed: r, 1/r (times take), and the
+// half step h = dq r / 2.
+struct PairGeom32 {
+    float r, amp, sh, chm1, lam;
+};
+
+__device__ __forceinline__ PairGeom32 pair_geom_f32(float r2, float hrev, float hlo, float take,
+                                                    bool small_h) {
     r2 = fmaxf(r2, 1e-12f);  // coincident atoms: the r -> 0 limit s_k -> q_k
     float r, rinv;
     sqrt_and_rsqrt(r2, r, rinv);
-    const float arev = __builtin_fmaf(q0rev, r, q0lo * r);
     const float hr = __builtin_fmaf(hrev, r, hlo * r);
-    float sa, ca, sh, chm1;
-    sincos_rev(arev, sa, ca);
+    PairGeom32 g;
     if (small_h) {
-        sincos_poly(hr, sh, chm1);
+        sincos_poly(hr, g.sh, g.chm1);
     } else {
         float ch;
-        sincos_rev(hr, sh, ch);
-        chm1 = ch - 1.0f;
+        sincos_rev(hr, g.sh, ch);
+        g.chm1 = ch - 1.0f;
     }
+    g.r = r;
+    g.amp = take * rinv;
+    g.lam = 4.0f * (g.sh * g.sh);
+    return g;
+}
+
+// The strip part: s_0 = sin(a)/r and delta_1 = 2 cos(a + h) sin(h)/r.
+__device__ __forceinline__ Seed32 seed_from_geom(const PairGeom32 &g, float q0rev, float q0lo) {
+    const float arev = __builtin_fmaf(q0rev, g.r, q0lo * g.r);
+    float sa, ca;
+    sincos_rev(arev, sa, ca);
     // cos(a + h) = cos a + (cos a (cos h - 1) - sin a sin h)
-    const float cah = ca + __builtin_fmaf(ca, chm1, -sa * sh);
-    const float amp = take * rinv;
+    const float cah = ca + __builtin_fmaf(ca, g.chm1, -sa * g.sh);
     Seed32 z;
-    z.s = sa * amp;
-    z.d = (2.0f * amp) * (cah * sh);
-    z.lam = 4.0f * (sh * sh);
+    z.s = sa * g.amp;
+    z.d = (2.0f * g.amp) * (cah * g.sh);
+    z.lam = g.lam;
     return z;
+}
+
+__device__ __forceinline__ Seed32 seed_f32(float r2, float q0rev, float q0lo, float hrev,
+                                           float hlo, float take, bool small_h) {
+    return seed_from_geom(pair_geom_f32(r2, hrev, hlo, take, small_h), q0rev, q0lo);
 }
 
 struct Seed64 {
@@ -599,6 +618,162 @@ __global__ void __launch_bounds__(256) debye_tiled_kernel(TiledArgs args) {
 }
 
 // ---------------------------------------------------------------------------
+// FP32, uniform grid, 128 q points per item as two 64-point halves.
+//   dry j class: per pair the geometry and the half-step terms (r, 1/r,
+//     sin h, cos h - 1, lambda) are computed once and seed both halves, each
+//     a 64-step recurrence into its own accumulators;
+//   hydrated j class: the halves run one after the other (acc0 = a, accw = b),
+//     with the 64-point pair block.
+// Every half-strip sees the same seeds, recurrence and carry as the 64-point
+// kernel, so the halves' partials equal that kernel's item partials.
+// ---------------------------------------------------------------------------
+template <bool MASK>
+__device__ __forceinline__ void dual_block_dry(const Atom<float> *__restrict__ atoms, int jb, int je,
+                                               const Atom<float> &ai, int iidx, float lane_take,
+                                               const float (&q0p)[2], const float (&q0lo)[2],
+                                               float hp, float hlo, bool small_h,
+                                               float (&acc_a)[64], float (&acc_b)[64]) {
+    const PRIPPS_CONST Atom<float> *ca = as_const(atoms);
+    Atom<float> next = load_atom(ca, jb);
+    for (int j = jb; j < je; ++j) {
+        const Atom<float> aj = next;
+        next = load_atom(ca, min(j + 1, je - 1));  // prefetch the next j atom
+        const float dx = ai.x - aj.x, dy = ai.y - aj.y, dz = ai.z - aj.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const float take = MASK ? (j > iidx ? 1.0f : 0.0f) : lane_take;
+        const PairGeom32 g = pair_geom_f32(r2, hp, hlo, take, small_h);
+        {
+            const Seed32 z = seed_from_geom(g, q0p[0], q0lo[0]);
+            float sv = z.s, d = z.d;
+#pragma unroll
+            for (int k = 0; k < 64; ++k) {
+                acc_a[k] += sv;
+                sv += d;
+                d -= z.lam * sv;
+            }
+        }
+        {
+            const Seed32 z = seed_from_geom(g, q0p[1], q0lo[1]);
+            float sv = z.s, d = z.d;
+#pragma unroll
+            for (int k = 0; k < 64; ++k) {
+                acc_b[k] += sv;
+                sv += d;
+                d -= z.lam * sv;
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) debye_dual_kernel(TiledArgs args) {
+    constexpr int Q = 128;
+    const int lane = __lane_id();
+    for (;;) {
+        int w = 0;
+        if (lane == 0) w = atomicAdd(args.counter, 1);
+        w = __builtin_amdgcn_readfirstlane(__shfl(w, 0, 64));
+        if (w >= args.n_items) break;
+
+        const Item it = load_item(args.items, w);
+        const PRIPPS_CONST DevStruct &sd = as_const(args.structs)[it.s];
+        const Atom<float> *__restrict__ atoms = static_cast<const Atom<float> *>(sd.atoms);
+        const int nq = sd.nq;
+
+        const bool ivalid = lane < it.icount;
+        const int iidx = ivalid ? it.i0 + lane : 0x7fffffff;
+        Atom<float> ai = atoms[ivalid ? it.i0 + lane : it.i0];
+        const float wi = ivalid ? ai.w : 0.0f;
+        const float lane_take = ivalid ? 1.0f : 0.0f;
+        const bool wet_i = __ballot(wi != 0.0f) != 0;
+        const bool small_h = sd.small_h != 0;
+
+        const int qbase = it.strip * Q;
+        float q0p[2], q0lo[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const double qs = sd.q0 + double(qbase + 64 * h) * sd.dq;
+            q0p[h] = float(qs * kInv2Pi);
+            q0lo[h] = float(qs * kInv2Pi - double(q0p[h]));
+        }
+        const double hd = sd.dq * (0.5 * kInv2Pi);
+        const float hp = float(hd);
+        const float hlo = float(hd - double(hp));
+        float qv_unused[64], ql_unused[64];
+#pragma unroll
+        for (int k = 0; k < 64; ++k) qv_unused[k] = ql_unused[k] = 0.0f;
+
+        // Lane owns q = qbase + 64 h + lane of each half h.  The FP64 profile
+        // accumulators live in the item's own partial slot (written only by
+        // this wave, L2-resident): a carry adds to it once per chunk, which
+        // keeps them and the form factors out of the register file.
+        double *out = args.partials + (int64_t)w * (kProfiles * Q);
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int p = 0; p < kProfiles; ++p) out[p * Q + 64 * h + lane] = 0.0;
+
+        auto carry = [&](int h, int lk, const float (&a0)[64], const float (&aw)[64], bool wet) {
+            float r0[1], ri[1], rw[1], riw[1];
+            reduce_scatter<float, 64, false>(a0, 1.0f, r0);
+            ri[0] = rw[0] = riw[0] = 0.0f;
+            if (wet_i) reduce_scatter<float, 64, true>(a0, wi, ri);
+            if (wet) {
+                reduce_scatter<float, 64, false>(aw, 1.0f, rw);
+                if (wet_i) reduce_scatter<float, 64, true>(aw, wi, riw);
+            }
+            const int m = min(qbase + 64 * h + lane, nq - 1);
+            const double ak = sd.ffa[it.ikind * nq + m];
+            const double ek = sd.ffe ? sd.ffe[it.ikind * nq + m] : 0.0;
+            const double wq = sd.ffw ? sd.ffw[m] : 0.0;
+            const double al = sd.ffa[lk * nq + m];
+            const double el = sd.ffe ? sd.ffe[lk * nq + m] : 0.0;
+            const double S = double(r0[0]), Si = double(ri[0]);
+            const double Sw = double(rw[0]), Siw = double(riw[0]);
+            double *o = out + 64 * h + lane;
+            o[0 * Q] += 2.0 * ak * al * S;
+            o[1 * Q] += 2.0 * ek * el * S;
+            o[2 * Q] += 2.0 * wq * wq * Siw;
+            o[3 * Q] += (ak * el + al * ek) * S;
+            o[4 * Q] += wq * (ak * Sw + al * Si);
+            o[5 * Q] += wq * (ek * Sw + el * Si);
+        };
+
+        int j = it.j0;
+        int c = 0;
+        while (j < it.j1) {
+            while (sd.cls_start[c + 1] <= j) ++c;
+            const int jend = min(min(j + sd.block, it.j1), sd.cls_start[c + 1]);
+            const int lk = sd.cls_kind[c];
+            const bool wet = sd.cls_wet[c] != 0;
+            const bool mask = j <= it.i0 + it.icount - 1;  // chunk meets the diagonal
+
+            float acc_a[64], acc_b[64];
+            if (!wet) {
+#pragma unroll
+                for (int k = 0; k < 64; ++k) acc_a[k] = acc_b[k] = 0.0f;
+                if (mask) dual_block_dry<true>(atoms, j, jend, ai, iidx, lane_take, q0p, q0lo, hp, hlo, small_h, acc_a, acc_b);
+                else dual_block_dry<false>(atoms, j, jend, ai, iidx, lane_take, q0p, q0lo, hp, hlo, small_h, acc_a, acc_b);
+                carry(0, lk, acc_a, acc_b, false);
+                carry(1, lk, acc_b, acc_a, false);
+            } else {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+#pragma unroll
+                    for (int k = 0; k < 64; ++k) acc_a[k] = acc_b[k] = 0.0f;
+                    if (mask)
+                        pair_block<float, 64, true, true, true, -1>(atoms, j, jend, ai, iidx, lane_take, qv_unused, ql_unused, q0p[h], q0lo[h], hp, hlo, small_h, acc_a, acc_b);
+                    else
+                        pair_block<float, 64, true, true, false, -1>(atoms, j, jend, ai, iidx, lane_take, qv_unused, ql_unused, q0p[h], q0lo[h], hp, hlo, small_h, acc_a, acc_b);
+                    carry(h, lk, acc_a, acc_b, true);
+                }
+            }
+            j = jend;
+        }
+
+    }
+}
+
+// ---------------------------------------------------------------------------
 // finalize: sum item partials per (structure, strip) in a fixed order, divide
 // by q, add the diagonal; q = 0 takes the closed form (sum f_X)(sum f_Y).
 // ---------------------------------------------------------------------------
@@ -852,7 +1027,10 @@ hipError_t launch_tiled(bool fp64, int Q, bool uniform, const TiledArgs &a, int 
                                     : launch_tiled_t<float, 32, false>(a, blocks, st);
         if (Q == 64) return uniform ? launch_tiled_t<float, 64, true>(a, blocks, st)
                                     : launch_tiled_t<float, 64, false>(a, blocks, st);
-        if (Q == 128 && uniform) return launch_tiled_t<float, 128, true>(a, blocks, st);
+        if (Q == 128 && uniform) {
+            hipLaunchKernelGGL(debye_dual_kernel, dim3(blocks), dim3(256), 0, st, a);
+            return hipGetLastError();
+        }
     } else {
         if (Q == 32) return uniform ? launch_tiled_t<double, 32, true>(a, blocks, st)
                                     : launch_tiled_t<double, 32, false>(a, blocks, st);
@@ -871,7 +1049,9 @@ hipError_t tiled_occupancy(bool fp64, int Q, bool uniform, int *blocks_per_cu) {
     if (!fp64) {
         if (Q == 32) return uniform ? occupancy_t<float, 32, true>(blocks_per_cu) : occupancy_t<float, 32, false>(blocks_per_cu);
         if (Q == 64) return uniform ? occupancy_t<float, 64, true>(blocks_per_cu) : occupancy_t<float, 64, false>(blocks_per_cu);
-        if (Q == 128 && uniform) return occupancy_t<float, 128, true>(blocks_per_cu);
+        if (Q == 128 && uniform)
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                blocks_per_cu, reinterpret_cast<const void *>(&debye_dual_kernel), 256, 0);
     } else {
         if (Q == 32) return uniform ? occupancy_t<double, 32, true>(blocks_per_cu) : occupancy_t<double, 32, false>(blocks_per_cu);
         if (Q == 64 && uniform) return occupancy_t<double, 64, true>(blocks_per_cu);

@@ -16,8 +16,16 @@ atom positions -- so a solvent model reduces to two per-q tables:
   (``src/solvent/sasa.rs:34-71``, voronota-ltr, not available here); this
   engine takes them explicitly (``hydration="weights"``).
 
-Grid and Voronoi dummy generators are not colocated with the atoms and are not
-offered (``src/solvent/grid.rs``, ``grid_excluded.rs``, ``voronoi_*.rs``).
+Grid dummies (``excluded="grid"``: cells inside the molecular envelope,
+``src/solvent/grid_excluded.rs:33-75``; ``hydration="grid"``: cells in a shell
+around it, ``src/solvent/grid.rs:40-95``) are not colocated with the atoms.
+They join the structure as extra points of their own kinds: an excluded-grid
+point carries only an excluded form factor, a hydration-grid point only a
+water weight, and an atom only its atom (and colocated excluded) form factor.
+The colocated-set identity over that union gives exactly the reference's six
+set sums -- self sums within each set and the full N x M cross sums between
+sets (``src/debye.rs:90-128``) -- so the same kernel serves both layouts.
+Voronoi dummies need the voronota-ltr tessellation, which is not available.
 """
 from __future__ import annotations
 
@@ -28,6 +36,8 @@ import numpy as np
 
 from .errors import UsageError, ValidationError
 from .formfactor import FormFactorMap
+
+WATER_SPACING = 3.10516              # src/solvent/grid.rs:85 (cube root of 30 A^3)
 
 WATER_KIND_NAME = "H2O"              # src/solvent/mod.rs:43
 WATER_VOLUME = 30.0                  # src/solvent/mod.rs:46
@@ -86,9 +96,13 @@ def excluded_table(model: str, ffmap: FormFactorMap, used_ids: Sequence[int], q:
                     "required by ExcludedVolume::PerKind")
             out[r] = [e.excluded_solvent.eval(float(x)) for x in q]
         return out
-    if model in ("grid", "voronoi"):
-        raise UsageError(f"excluded-volume model `{model}` places dummies away from the atoms; "
-                         "the Debye engine supports `none`, `fraser`, `foxs` and `per-kind`")
+    if model == "grid":
+        # not colocated: the atoms carry no excluded form factor (grid_dummies)
+        return np.zeros((len(used_ids), len(q)))
+    if model == "voronoi":
+        raise UsageError("excluded-volume model `voronoi` needs the voronota-ltr tessellation, "
+                         "which this engine does not ship; use `grid`, `fraser`, `foxs` or "
+                         "`per-kind`")
     raise ValueError(f"unknown excluded-volume model: {model} (expected `none`, `fraser`, "
                      "`foxs`, `grid`, `voronoi`, or `per-kind`)")
 
@@ -109,8 +123,115 @@ def water_table(model: str, ffmap: FormFactorMap, q: np.ndarray, weights,
             raise ValidationError(f"Hydration::Sasa requires an `{WATER_KIND_NAME}` entry in the "
                                   "form-factor YAML")
         return np.array([water.eval(float(x)) for x in q]), w
-    if model in ("sasa", "grid", "voronoi"):
+    if model == "grid":
+        # not colocated: the atoms carry no hydration weight (grid_dummies)
+        return water_form_factor(ffmap, q, "Hydration::Grid"), np.zeros(n_atoms)
+    if model in ("sasa", "voronoi"):
         raise UsageError(f"hydration model `{model}` needs a surface tessellation that this engine "
-                         "does not ship; pass hydration='weights' with per-atom weights")
+                         "does not ship; pass hydration='weights' with per-atom weights, or "
+                         "hydration='grid'")
     raise ValueError(f"unknown hydration model: {model} (expected `none`, `sasa`, `grid`, "
                      "`voronoi`, or `weights`)")
+
+
+def water_form_factor(ffmap: FormFactorMap, q: np.ndarray, caller: str) -> np.ndarray:
+    """The shared ``H2O`` form factor at q (src/solvent/mod.rs:83-93)."""
+    water = ffmap.form_factor_by_name(WATER_KIND_NAME)
+    if water is None:
+        raise ValidationError(f"{caller} requires an `{WATER_KIND_NAME}` entry in the "
+                              "form-factor YAML")
+    return np.array([water.eval(float(x)) for x in q])
+
+
+def atom_radii(ffmap: FormFactorMap, ids: np.ndarray, caller: str) -> np.ndarray:
+    """Per-atom radius = sigma / 2 (the YAML ``radius``; src/solvent/mod.rs:59-81)."""
+    out = np.empty(len(ids))
+    by_kind = {}
+    for i, kid in enumerate(np.asarray(ids).tolist()):
+        r = by_kind.get(kid)
+        if r is None:
+            kind = ffmap.entries[kid].kind
+            if kind.sigma is None:
+                raise ValidationError(f"atom kind {kind.name} has no `radius`; required by {caller}")
+            r = by_kind[kid] = kind.sigma * 0.5
+        out[i] = r
+    return out
+
+
+def _grid_points(pos: np.ndarray, margin: float, spacing: float) -> np.ndarray:
+    """Cartesian grid over the bounding box widened by `margin`: per axis
+    ceil((hi - lo) / spacing) + 1 points lo + i spacing, x slowest
+    (src/solvent/grid.rs:34-37, 52-67)."""
+    lo = pos.min(axis=0) - margin
+    hi = pos.max(axis=0) + margin
+    axes = [lo[a] + np.arange(int(math.ceil((hi[a] - lo[a]) / spacing)) + 1, dtype=np.float64)
+            * spacing for a in range(3)]
+    gx, gy, gz = np.meshgrid(axes[0], axes[1], axes[2], indexing="ij")
+    return np.stack([gx.ravel(), gy.ravel(), gz.ravel()], axis=1)
+
+
+def nearest_surface_distance(points: np.ndarray, centers: np.ndarray, radii: np.ndarray,
+                             cutoff: float) -> np.ndarray:
+    """min_j (|p - c_j| - r_j) for every point (src/solvent/grid.rs:24-30),
+    with |.| evaluated as sqrt(dx^2 + dy^2 + dz^2) like the reference.  Only
+    atoms with |p - c_j| <= cutoff are examined (a k-d tree finds them);
+    points with none get +inf.  Callers pick `cutoff` so that no atom beyond
+    it can change their criterion."""
+    from scipy.spatial import cKDTree
+    out = np.full(len(points), np.inf)
+    if len(points) == 0 or len(centers) == 0:
+        return out
+    tree = cKDTree(centers)
+    step = 1 << 16
+    for a in range(0, len(points), step):
+        P = points[a:a + step]
+        pairs = cKDTree(P).sparse_distance_matrix(tree, cutoff * (1.0 + 1e-9) + 1e-9,
+                                                  output_type="ndarray")
+        if len(pairs) == 0:
+            continue
+        i, j = pairs["i"], pairs["j"]
+        d = P[i] - centers[j]
+        dist = np.sqrt(d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]) - radii[j]
+        order = np.argsort(i, kind="stable")
+        i, dist = i[order], dist[order]
+        starts = np.flatnonzero(np.r_[True, i[1:] != i[:-1]])
+        out[a + i[starts]] = np.minimum.reduceat(dist, starts)
+    return out
+
+
+def grid_excluded_dummies(pos: np.ndarray, radii: np.ndarray, spacing: float) -> np.ndarray:
+    """Grid cells inside the molecular envelope (nearest surface distance
+    <= 0), src/solvent/grid_excluded.rs:33-75."""
+    if len(pos) == 0:
+        return np.zeros((0, 3))
+    max_r = max(0.0, float(radii.max()))
+    pts = _grid_points(pos, max_r + spacing, spacing)
+    d = nearest_surface_distance(pts, pos, radii, max_r)
+    return pts[d <= 0.0]
+
+
+def grid_excluded_form_factor(q: np.ndarray, rho: float, spacing: float) -> np.ndarray:
+    """Fraser Gaussian of one cell, V = spacing^3, PepsiSAXS width
+    (src/solvent/grid_excluded.rs:71-74): rho V exp(-V^(2/3) q^2 / (4 pi))."""
+    v = spacing ** 3
+    a = rho * v
+    b = v ** (2.0 / 3.0) / PEPSI_DENOMINATOR
+    return np.array([a * math.exp(-b * float(x) * float(x)) for x in q])
+
+
+def grid_hydration_dummies(pos: np.ndarray, radii: np.ndarray, spacing: float, thickness: float,
+                           probe: float) -> np.ndarray:
+    """Grid cells in the shell probe < d < probe + thickness around the
+    envelope, src/solvent/grid.rs:40-95."""
+    if len(pos) == 0:
+        return np.zeros((0, 3))
+    max_r = max(0.0, float(radii.max()))
+    outer = probe + thickness
+    pts = _grid_points(pos, max_r + probe + thickness + spacing, spacing)
+    d = nearest_surface_distance(pts, pos, radii, max_r + outer)
+    return pts[(d > probe) & (d < outer)]
+
+
+def molecules_per_cell(spacing: float) -> float:
+    """Water molecules per hydration cell, (spacing / 3.10516)^3 (src/solvent/grid.rs:80-92)."""
+    return (spacing / WATER_SPACING) ** 3

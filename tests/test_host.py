@@ -136,8 +136,9 @@ def test_solvent_tables():
         b = v ** (2.0 / 3.0) / FOXS_DENOMINATOR
         assert abs(t[r, 7] - 0.334 * v * math.exp(-b * q[7] ** 2)) < 1e-14
     assert excluded_table("none", ff, used, q, 0.334) is None
-    with pytest.raises(E.UsageError):
-        excluded_table("grid", ff, used, q, 0.334)
+    with pytest.raises(E.UsageError, match="voronota"):
+        excluded_table("voronoi", ff, used, q, 0.334)
+    assert np.all(excluded_table("grid", ff, used, q, 0.334) == 0.0)  # cells carry it
     with pytest.raises(ValueError, match="unknown excluded-volume model"):
         excluded_table("bogus", ff, used, q, 0.334)
     wf, w = water_table("weights", ff, q, np.ones(3), 3)
