@@ -33,11 +33,15 @@ UNIT = "pair·q terms/s"
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
 FLOP_PER_TERM_SOLVENT = 14.0
-# HBM bytes per launch of the pair kernel on the default solvent10k plan (Q = 64,
-# span = 448, 14,760 items), from rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
-# (scale factors from a plain-copy calibration); profiles/debye_tiled_f32_solvent10k.md
-TRAFFIC_SOLVENT10K = {"q_per_lane": 64, "span": 448, "work_items": 14760,
-                      "bytes": (2379 * 2.0 + 44805 * 1.0) * 1024}
+# Counter figures of the pair kernel on the default solvent10k plan (Q = 64,
+# span = 256, 25,344 items), from profiles/debye_tiled_f32_solvent10k.md:
+#   bytes: HBM bytes per launch, rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
+#          (scale factors from a plain-copy calibration);
+#   executed_flop_per_term: rocprofv3 TOTAL_32_OPS ((2 FMA + ADD + MUL + TRANS +
+#          INT32) x 64 per dispatch) / pair-q terms.
+COUNTERS_SOLVENT10K = {"q_per_lane": 64, "span": 256, "work_items": 25344,
+                       "bytes": (3051.3 * 2.0 + 76888.0 * 1.0) * 1024,
+                       "executed_flop_per_term": 1.552e11 / 2.49975e10}
 PAIRS = ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))
 
 
@@ -77,6 +81,9 @@ def main():
                     help="batch512: structures timed on the sequential plain FP64 path")
     ap.add_argument("--rates", default="10,50,100,200,500,1000,2000")
     ap.add_argument("--requests", type=int, default=100, help="served: requests per rate")
+    ap.add_argument("--total-requests", type=int, default=0,
+                    help="served: requests over the whole sweep, in proportion to the rate "
+                         "(0: --requests per rate)")
     ap.add_argument("--served-rate", type=float, default=200.0,
                     help="default run: offered rate of its short served stream")
     ap.add_argument("--served-requests", type=int, default=200,
@@ -171,12 +178,18 @@ def main():
             f"fp32 vs oracle {err_oracle:.2e}")
 
     plan = {"q_per_lane": st.q_per_lane, "span": st.span, "work_items": st.work_items}
-    traffic = (TRAFFIC_SOLVENT10K["bytes"]
-               if all(TRAFFIC_SOLVENT10K[k] == v for k, v in plan.items()) else None)
+    same_plan = all(COUNTERS_SOLVENT10K[k] == v for k, v in plan.items())
+    traffic = COUNTERS_SOLVENT10K["bytes"] if same_plan else None
+    executed = None
+    if same_plan:
+        tf = COUNTERS_SOLVENT10K["executed_flop_per_term"] * terms / (pair_ms * 1e-3) / 1e12
+        executed = {"flop_per_term": COUNTERS_SOLVENT10K["executed_flop_per_term"],
+                    "achieved": tf, "frac": tf / FP32_PEAK_TFLOPS,
+                    "source": "rocprofv3 TOTAL_32_OPS of the same plan"}
 
     served = None
     if args.served_requests > 0:
-        rec = _served_streams(eng, p, [args.served_rate], args.served_requests)[0]
+        rec = _served_streams(eng, p, [args.served_rate], [args.served_requests])[0]
         served = {"workload": SERVED_WORKLOAD, **rec}
 
     out = {
@@ -200,7 +213,8 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": "debye_tiled_kernel<float,%d,true>" % st.q_per_lane,
-                     "kernel_ms": pair_ms, "flop_per_term": FLOP_PER_TERM_SOLVENT},
+                     "kernel_ms": pair_ms, "flop_per_term": FLOP_PER_TERM_SOLVENT,
+                     "executed": executed},
         "cpu_baseline": cpu,
         "max_rel_err_vs_fp64_gpu": err_fp64,
         "max_rel_err_vs_oracle_sampled_q": err_oracle,
@@ -378,13 +392,18 @@ def bench_batch512(args):
 
 
 def _request_job(job):
-    from pripps_amd.synth import globule, radius_for
+    """One request's globule; elements come back as indices into synth.ELEMENTS."""
+    import numpy as np
+    from pripps_amd.synth import ELEMENTS, globule, radius_for
     n, seed = job
-    return globule(n, radius_for(n), seed, solvent=True)
+    names, pos, w = globule(n, radius_for(n), seed, solvent=True)
+    idx = np.array([ELEMENTS.index(e) for e in names], dtype=np.int8)
+    return idx, pos, w
 
 
-def _served_streams(eng, ff, rates, requests):
-    """Poisson request streams through the serving loop, one stream per rate.
+def _served_streams(eng, ff, rates, counts):
+    """Poisson request streams through the serving loop, one stream per rate
+    (counts[r] requests at rates[r]).
 
     Each request is a fresh seeded globule (N log-uniform in [300, 30000], nq
     uniform in [200, 800], q on [0, 0.5], solvent on) followed by five
@@ -393,14 +412,20 @@ def _served_streams(eng, ff, rates, requests):
     import numpy as np
     from concurrent.futures import ProcessPoolExecutor, ThreadPoolExecutor
     from pripps_amd.server import Server
-    from pripps_amd.synth import Workload, linear_q, request_stream
-    total = requests * len(rates)
+    from pripps_amd.synth import ELEMENTS, linear_q, request_stream
+    total = int(sum(counts))
     ns, nqs, knobs, seeds = request_stream(total)
     t = time.perf_counter()
+    lut = ff.names_to_ids(list(ELEMENTS))
+    inps = []
+    jobs = [(int(ns[i]), int(seeds[i])) for i in range(total)]
     with ProcessPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
-        gl = list(ex.map(_request_job, [(int(ns[i]), int(seeds[i])) for i in range(total)]))
-    inps = [ff.prepare_workload(Workload(f"r{i}", gl[i][0], gl[i][1], linear_q(0.0, 0.5, int(nqs[i])), gl[i][2]))
-            for i in range(total)]
+        for i, (idx, pos, w) in enumerate(ex.map(_request_job, jobs, chunksize=8)):
+            inps.append(ff.prepare_input(pos, lut[idx], linear_q(0.0, 0.5, int(nqs[i])),
+                                         excluded="foxs", hydration="weights",
+                                         hydration_weights=w))
+            if (i + 1) % 1000 == 0:
+                log(f"served: prepared {i + 1}/{total} requests ({time.perf_counter() - t:.0f} s)")
     terms = np.array([ns[i] * (ns[i] - 1) / 2 * nqs[i] for i in range(total)], dtype=float)
     log(f"served: {total} requests prepared in {time.perf_counter() - t:.1f} s, "
         f"mean {terms.mean():.3e} terms/request")
@@ -418,8 +443,12 @@ def _served_streams(eng, ff, rates, requests):
     with Server(eng, "fp32") as srv, ThreadPoolExecutor(max_workers=64) as clients:
         for i in range(2):  # warm both lanes
             srv.wait(srv.submit(inps[i]))
-        for ri, rate in enumerate(rates):
-            idx = list(range(ri * requests, (ri + 1) * requests))
+        start = 0
+        for rate, count in zip(rates, counts):
+            idx = list(range(start, start + int(count)))
+            start += int(count)
+            if not idx:
+                continue
             arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(idx)))
             t0 = time.perf_counter()
             futs = []
@@ -440,6 +469,8 @@ def _served_streams(eng, ff, rates, requests):
                    "build_p99_ms": float(np.percentile(build, 99))}
             results.append(rec)
             log(f"served rate {rate}: {rec}")
+            for i in idx:
+                inps[i] = None  # release host copies as the sweep goes
     return results
 
 
@@ -450,9 +481,17 @@ SERVED_WORKLOAD = ("served: Poisson request streams, globules N log-U[300,30000]
 
 def bench_served(args):
     """Poisson request streams at several rates through the serving loop."""
+    import numpy as np
     eng, ff = _engine_and_ff()
     rates = [float(r) for r in args.rates.split(",")]
-    results = _served_streams(eng, ff, rates, args.requests)
+    if args.total_requests > 0:
+        # requests in proportion to the rate, so every stream lasts about as long
+        share = np.array(rates) / sum(rates) * args.total_requests
+        counts = np.floor(share).astype(int)
+        counts[np.argsort(counts - share)[: args.total_requests - counts.sum()]] += 1
+    else:
+        counts = [args.requests] * len(rates)
+    results = _served_streams(eng, ff, rates, [int(c) for c in counts])
     print(json.dumps({"workload": SERVED_WORKLOAD, "rates": results}), flush=True)
 
 

@@ -179,9 +179,12 @@ class Pripps:
         hy = hydration_name(hydration)
         ids = np.asarray(ids, dtype=np.int64)
         pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
-        used = sorted(set(int(i) for i in ids)) or [0]
-        remap = {k: r for r, k in enumerate(used)}
-        kind = np.array([remap[int(i)] for i in ids], dtype=np.int32)
+        if len(ids):
+            used_arr, inverse = np.unique(ids, return_inverse=True)
+            used = [int(k) for k in used_arr]
+            kind = inverse.astype(np.int32).reshape(-1)
+        else:
+            used, kind = [0], np.zeros(0, dtype=np.int32)
         q = np.ascontiguousarray(q, dtype=np.float64)
         atom_ff = self.formfactors.table(used, q)
         excl = excluded_table(ex, self.formfactors, used, q, bulk_electron_density)
