@@ -79,6 +79,10 @@ def lib():
         L.oracle_fit_grid.argtypes = [THEORY_FN, ctypes.c_void_p, _D, ctypes.c_int64, _D, _D, _D,
                                       ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _D, _D, _D]
         L.oracle_fit_grid.restype = ctypes.c_int
+        L.oracle_direct_rows.argtypes = [ctypes.c_int64, _D, _D, _D, _I32, ctypes.c_int32, _D,
+                                         ctypes.c_int32, _I32, _D, _D, _D, _D, ctypes.c_int32,
+                                         ctypes.c_int, _D]
+        L.oracle_direct_rows.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -261,3 +265,39 @@ def fit(theory, tq, q, inten, sigma=None, method="nelder-mead", c1_points=151, c
     if rc != 0:
         raise ValueError(FIT_ERRORS.get(rc, f"fit error {rc}"))
     return tuple(out)
+
+
+# ---------------------------------------------------------------------------
+# Direct Fourier transform (direct_oracle.c)
+# ---------------------------------------------------------------------------
+DIRECT_REFERENCE = 0
+DIRECT_TABLE = 1
+DIRECT_EXTENDED = 2
+
+
+def direct_rows(pos, kind, box, indices, atom_ff, excluded_ff=None, water_ff=None, weights=None,
+                num_molecules=0, mode=DIRECT_EXTENDED):
+    """Per q-vector [n_vectors][7]: Re A, Im A, Re E, Im E, Re H, Im H, self
+    (direct_oracle.c: src/explicit.rs:136-146, 217-244, 283-326)."""
+    pos = np.ascontiguousarray(np.asarray(pos, dtype=np.float64).reshape(-1, 3))
+    x, y, z = (np.ascontiguousarray(pos[:, d]) for d in range(3))
+    kind = np.ascontiguousarray(kind, dtype=np.int32)
+    idx = np.ascontiguousarray(np.asarray(indices, dtype=np.int32).reshape(-1, 3))
+    V = len(idx)
+    box = np.ascontiguousarray(np.broadcast_to(np.asarray(box, dtype=np.float64), (3,)))
+    ffa = np.ascontiguousarray(atom_ff, dtype=np.float64)
+    ffe = None if excluded_ff is None else np.ascontiguousarray(excluded_ff, dtype=np.float64)
+    ffw = None if water_ff is None else np.ascontiguousarray(water_ff, dtype=np.float64)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    K = ffa.shape[0] if ffa.ndim == 2 else 1
+    out = np.zeros((V, 7))
+
+    def ptr(a):
+        return None if a is None else a.ctypes.data_as(_D)
+
+    rc = lib().oracle_direct_rows(len(x), ptr(x), ptr(y), ptr(z), kind.ctypes.data_as(_I32), K,
+                                  ptr(box), V, idx.ctypes.data_as(_I32), ptr(ffa), ptr(ffe),
+                                  ptr(ffw), ptr(w), int(num_molecules), int(mode), ptr(out))
+    if rc != 0:
+        raise MemoryError("oracle_direct_rows")
+    return out

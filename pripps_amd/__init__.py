@@ -76,12 +76,17 @@ class FitMethod(enum.Enum):
 @dataclass
 class Intensity:
     """I(q) and the unscaled self terms (reference ``Intensity`` /
-    ``Contributions``, src/lib.rs:89-121)."""
+    ``Contributions``, src/lib.rs:89-121).  ``p`` and ``s_eff`` are the
+    single-molecule form factor P(q) and the effective structure factor
+    S_eff(q) of a Direct run with ``num_molecules`` (src/lib.rs:511-526;
+    pripps-py ``Intensity.p`` / ``.s_eff``)."""
     q: np.ndarray
     total: np.ndarray
     atoms: Optional[np.ndarray] = None
     excluded: Optional[np.ndarray] = None
     hydration: Optional[np.ndarray] = None
+    p: Optional[np.ndarray] = None
+    s_eff: Optional[np.ndarray] = None
 
     def __len__(self):
         return len(self.q)
@@ -145,9 +150,77 @@ class DebyeModel:
         return self._native.stats()
 
 
+class Scheme(enum.Enum):
+    """Intensity scheme (pripps-py/src/lib.rs:45-50).  Multipole is the
+    reference's default and is not part of this engine."""
+    Multipole = "multipole"
+    Debye = "debye"
+    Direct = "direct"
+
+
+def _scheme_name(model) -> str:
+    if model is None:
+        return "multipole"
+    name = getattr(model, "value", model)
+    if not isinstance(name, str):
+        raise TypeError("`model` must be a str or pripps.Scheme")
+    name = name.lower()
+    if name not in ("multipole", "debye", "direct"):
+        raise ValueError(f"unknown model: {name} (expected `multipole`, `debye`, or `direct`)")
+    return name
+
+
+class DirectModel:
+    """A cached Direct model: per q-vector amplitudes of atoms, excluded volume
+    and hydration resident on the GPU (reference ``DirectModel``,
+    src/explicit.rs:328-366)."""
+
+    def __init__(self, native, engine: Engine, n_vectors: int):
+        self._native = native
+        self._engine = engine
+        self._n_vectors = n_vectors
+        self._profile_model = None
+
+    @property
+    def q_values(self) -> np.ndarray:
+        return self._native.q_values()
+
+    def intensity(self, volume_scale: float = 1.0, contrast_density: float = 0.03) -> Intensity:
+        total, atoms, excluded, hydration = self._native.intensity(volume_scale, contrast_density)
+        return Intensity(self.q_values, total, atoms, excluded, hydration)
+
+    def profiles(self) -> np.ndarray:
+        """Group means [6][n_groups] of |A|^2, |E|^2, |H|^2, Re A E*, Re A H*,
+        Re E H*: the Debye combine of these is ``intensity(vs, cd).total``."""
+        return self._native.profiles()
+
+    def amplitudes(self) -> np.ndarray:
+        """Per q-vector [n_vectors][7]: Re A, Im A, Re E, Im E, Re H, Im H, self."""
+        return self._native.amplitudes(self._n_vectors)
+
+    def self_term(self) -> np.ndarray:
+        return self._native.self_term()
+
+    def _fit_model(self) -> NativeModel:
+        if self._profile_model is None:
+            self._profile_model = self._engine.from_profiles(self.q_values, self.profiles())
+        return self._profile_model
+
+    def fit(self, q, intensity, sigma=None, *, method=None, c1_points: int = 151,
+            c2_points: int = 101, c1_bounds=None, c2_bounds=None) -> FitResult:
+        """Bounded (volume_scale, contrast_density) fit, as the reference fits a
+        DirectModel through its ``intensity`` closure (pripps-py/src/lib.rs:625-650)."""
+        r = self._fit_model().fit(q, intensity, sigma, _fit_method(method), c1_points, c2_points,
+                                  c1_bounds, c2_bounds)
+        return FitResult(r.volume_scale, r.contrast_density, r.chi2, r.scale)
+
+    def stats(self):
+        return self._native.stats()
+
+
 class Pripps:
-    """Form-factor set plus the Debye entry point (reference ``Pripps``,
-    src/lib.rs:226-303; Python class pripps-py/src/lib.rs)."""
+    """Form-factor set plus the Debye and Direct entry points (reference
+    ``Pripps``, src/lib.rs:226-528; Python class pripps-py/src/lib.rs)."""
 
     def __init__(self, atomfile: Union[str, Sequence[str]], *, engine: Optional[Engine] = None):
         if isinstance(atomfile, (list, tuple)):
@@ -316,6 +389,122 @@ class Pripps:
                                             precision=precision, tuning=tuning,
                                             max_frames_per_launch=max_frames_per_launch)
         return DebyeModel(native, q)
+
+    # -- Direct Fourier transform -------------------------------------------
+    def direct_model(self, structure_file, *, pmax: int = 6, side_length: float = 200.0,
+                     excluded=None, hydration=None, probe: float = 1.8,
+                     spacing: float = 3.10516, thickness: float = 3.0,
+                     excluded_spacing: float = 1.0, bulk_electron_density: float = 0.334,
+                     box_sides=None, grid=None, hydration_weights=None,
+                     precision: str = "fp32") -> DirectModel:
+        """Cached Direct model in a periodic box (reference
+        ``Pripps::direct_model``, src/lib.rs:305-331; pripps-py/src/lib.rs:441-483):
+        Miller q-vectors p = 1..pmax in a cubic box of ``side_length`` A.
+        Extensions: ``box_sides`` (an orthorhombic box), ``grid`` (a
+        ``pripps_amd.direct.QGrid``, e.g. a lattice grid), ``hydration_weights``,
+        ``precision`` ("fp32" or "fp64")."""
+        st = structure_from_file(structure_file, self.formfactors.kinds())
+        return self.direct_model_from_arrays(
+            st.pos, st.ids, pmax=pmax, box_sides=side_length if box_sides is None else box_sides,
+            excluded=excluded, hydration=hydration, probe=probe, spacing=spacing,
+            thickness=thickness, excluded_spacing=excluded_spacing,
+            bulk_electron_density=bulk_electron_density, grid=grid,
+            hydration_weights=hydration_weights, precision=precision)
+
+    def direct_model_from_arrays(self, pos, ids, *, pmax: int = 6, box_sides=200.0,
+                                 excluded=None, hydration=None, probe=1.8, spacing=3.10516,
+                                 thickness=3.0, excluded_spacing=1.0, bulk_electron_density=0.334,
+                                 grid=None, hydration_weights=None, num_molecules: int = 0,
+                                 precision="fp32") -> DirectModel:
+        from .direct import QGrid, build_native, check_box, parse_box
+        box = parse_box(box_sides)
+        check_box(box)
+        grid = QGrid.miller(pmax) if grid is None else grid
+        idx, qn, group, group_q = grid.groups(box)
+        inp = self.prepare_input(pos, ids, qn, excluded=excluded, hydration=hydration,
+                                 hydration_weights=hydration_weights,
+                                 bulk_electron_density=bulk_electron_density, probe=probe,
+                                 spacing=spacing, thickness=thickness,
+                                 excluded_spacing=excluded_spacing)
+        native = build_native(self.engine, inp, box, idx, group, group_q,
+                              num_molecules=num_molecules, precision=precision)
+        return DirectModel(native, self.engine, len(idx))
+
+    def intensity(self, structure_file, *, model=None, xtc=None, stride: int = 1, weights=None,
+                  box_sides=None, excluded=None, hydration=None, qmin: float = 0.0,
+                  qmax: float = 0.5, nq=None, l_max=None, pmax: int = 6, probe: float = 1.8,
+                  spacing: float = 3.10516, thickness: float = 3.0,
+                  excluded_spacing: float = 1.0, bulk_electron_density: float = 0.334,
+                  volume_scale: float = 1.0, contrast_density: float = 0.03,
+                  num_molecules=None, hydration_weights=None, grid=None,
+                  precision: str = "fp32") -> Intensity:
+        """One-shot I(q) (reference ``Pripps::intensity``, src/lib.rs:333-528;
+        pripps-py/src/lib.rs:485-595) for ``model="debye"`` or ``"direct"``.
+        The Multipole scheme and XTC input are not part of this engine."""
+        from .direct import parse_box
+        scheme = _scheme_name(model)
+        if scheme == "multipole":
+            raise UsageError("the multipole scheme is not part of this engine; pass "
+                             "model='debye' or model='direct'")
+        if xtc is not None:
+            raise UsageError("XTC trajectories are not read by this engine; pass decoded frames "
+                             "to Pripps.trajectory_average")
+        if int(stride) < 1:
+            raise ValueError("`stride` must be >= 1")
+        st = structure_from_file(structure_file, self.formfactors.kinds())
+        ex, hy = excluded_name(excluded), hydration_name(hydration)
+        solvent_active = ex != "none" or hy != "none"
+        if num_molecules is not None:
+            n = int(num_molecules)
+            if scheme != "direct":
+                raise UsageError("--num-molecules requires the `direct` scheme (the periodic-box "
+                                 "method)")
+            if n == 0:
+                raise UsageError("--num-molecules must be at least 1")
+            if solvent_active:
+                raise UsageError("--num-molecules supports vacuum runs only (no "
+                                 "excluded-volume/hydration model)")
+            if len(st.ids) % n != 0:
+                raise ValidationError(f"box atom count ({len(st.ids)}) is not a multiple of "
+                                      f"--num-molecules ({n})")
+            m = len(st.ids) // n
+            if m > 0 and any(not np.array_equal(st.ids[b * m:(b + 1) * m], st.ids[:m])
+                             for b in range(n)):
+                raise ValidationError(f"--num-molecules {n} does not partition the box into {n} "
+                                      "identical contiguous molecules: the atom composition "
+                                      "differs between blocks")
+        if scheme == "debye":
+            dm = self.debye_model_from_arrays(
+                st.pos, st.ids, qmin=qmin, qmax=qmax, nq=100 if nq is None else nq,
+                excluded=excluded, hydration=hydration,
+                bulk_electron_density=bulk_electron_density,
+                hydration_weights=hydration_weights, precision=precision, probe=probe,
+                spacing=spacing, thickness=thickness, excluded_spacing=excluded_spacing)
+            res = dm.intensity(volume_scale, contrast_density)
+            if not solvent_active:
+                res.atoms = res.excluded = res.hydration = None
+            return res
+        if box_sides is None:
+            raise UsageError("direct Fourier transform requires box side lengths on the "
+                             "structure")
+        box = parse_box(box_sides)
+        dmod = self.direct_model_from_arrays(
+            st.pos, st.ids, pmax=pmax, box_sides=box, excluded=excluded, hydration=hydration,
+            probe=probe, spacing=spacing, thickness=thickness, excluded_spacing=excluded_spacing,
+            bulk_electron_density=bulk_electron_density, grid=grid,
+            hydration_weights=hydration_weights,
+            num_molecules=0 if num_molecules is None else int(num_molecules), precision=precision)
+        if solvent_active:
+            return dmod.intensity(volume_scale, contrast_density)
+        total = dmod.intensity(1.0, 0.0).total
+        res = Intensity(dmod.q_values, total)
+        if num_molecules is not None:
+            self_term = dmod.self_term()
+            res.p = self_term / float(int(num_molecules))
+            with np.errstate(divide="ignore", invalid="ignore"):
+                res.s_eff = np.where(self_term > 0.0, total / np.where(self_term > 0, self_term, 1.0),
+                                     np.nan)
+        return res
 
     def names_to_ids(self, names: Sequence[str]) -> np.ndarray:
         return resolve_ids(names, self.formfactors.kinds())

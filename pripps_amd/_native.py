@@ -1,4 +1,4 @@
-"""ctypes binding of the C ABI in include/pripps_debye.h.
+"""ctypes binding of the C ABI in include/pripps_debye.h and include/pripps_direct.h.
 
 The shared library ``pripps_amd/libpripps_debye.so`` is built in-tree by
 ``make -C pripps_amd/csrc`` (``__graft_entry__.build()``).  There is no
@@ -73,12 +73,18 @@ class TuningEntry(ctypes.Structure):
                 ("pair_ms", ctypes.c_double)]
 
 
+class DirectGrid(ctypes.Structure):
+    _fields_ = [("box", ctypes.c_double * 3), ("n_vectors", ctypes.c_int32),
+                ("indices", _I32), ("num_molecules", ctypes.c_int32),
+                ("n_groups", ctypes.c_int32), ("group", _I32), ("group_q", _D)]
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("max_in_flight", ctypes.c_int32),
                 ("memory_budget", ctypes.c_uint64)]
 
 
-# every exported symbol of include/pripps_debye.h
+# every exported symbol of include/pripps_debye.h and include/pripps_direct.h
 EXPORTS = (
     "pripps_last_error", "pripps_version", "pripps_engine_create", "pripps_engine_destroy",
     "pripps_engine_memory", "pripps_debye_model_build", "pripps_debye_model_build_batch",
@@ -89,6 +95,10 @@ EXPORTS = (
     "pripps_debye_model_free", "pripps_engine_autotune", "pripps_engine_tuning_table",
     "pripps_engine_set_tuning_entry", "pripps_debye_frames_average", "pripps_server_create",
     "pripps_server_submit", "pripps_server_wait", "pripps_server_destroy",
+    "pripps_direct_model_build", "pripps_direct_model_n_groups", "pripps_direct_model_q_values",
+    "pripps_direct_model_intensity", "pripps_direct_model_self_term",
+    "pripps_direct_model_profiles", "pripps_direct_model_amplitudes", "pripps_direct_model_stats",
+    "pripps_direct_model_recompute", "pripps_direct_model_free",
 )
 
 _lib = None
@@ -150,6 +160,20 @@ def load():
         L.pripps_server_wait.argtypes = [vp, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(vp),
                                          ctypes.POINTER(ctypes.c_double)]
         L.pripps_server_destroy.argtypes = [vp]
+        L.pripps_direct_model_build.argtypes = [vp, ctypes.POINTER(DebyeInput),
+                                                ctypes.POINTER(DirectGrid), ctypes.c_int32,
+                                                ctypes.POINTER(vp)]
+        L.pripps_direct_model_n_groups.argtypes = [vp, ctypes.POINTER(ctypes.c_int32)]
+        L.pripps_direct_model_q_values.argtypes = [vp, _D]
+        L.pripps_direct_model_intensity.argtypes = [vp, ctypes.c_double, ctypes.c_double, _D, _D,
+                                                    _D, _D]
+        L.pripps_direct_model_self_term.argtypes = [vp, _D]
+        L.pripps_direct_model_profiles.argtypes = [vp, _D]
+        L.pripps_direct_model_amplitudes.argtypes = [vp, _D]
+        L.pripps_direct_model_stats.argtypes = [vp, ctypes.POINTER(BuildStats)]
+        L.pripps_direct_model_recompute.argtypes = [vp, ctypes.c_int32,
+                                                    ctypes.POINTER(ctypes.c_double)]
+        L.pripps_direct_model_free.argtypes = [vp]
         for name in EXPORTS:
             if name not in ("pripps_last_error", "pripps_version"):
                 getattr(L, name).restype = ctypes.c_int

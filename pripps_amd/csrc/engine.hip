@@ -1484,4 +1484,12 @@ int engine_build(pripps_engine *e, int32_t n, const pripps_debye_input *inputs, 
 }
 void set_last_error(const std::string &msg) { g_last_error = msg; }
 const std::string &get_last_error() { return g_last_error; }
+
+// used by the Direct transform (direct.hip): the engine's device, stream and
+// submission lock; a model keeps its engine alive through retain/release.
+EngineAccess engine_access(pripps_engine *e) {
+    return EngineAccess{e->device, e->n_cu, e->stream, e->ev0, e->ev1, &e->mu};
+}
+void engine_retain(pripps_engine *e) { e->refs.fetch_add(1); }
+void engine_unref(pripps_engine *e) { engine_release(e); }
 }  // namespace pripps

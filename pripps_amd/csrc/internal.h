@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 namespace pripps {
 
 // Maximum number of distinct atom kinds in one structure; classes are
@@ -84,6 +86,15 @@ struct PlainArgs {
     const int4 *tiles;  // (i0, icount, ikind, 0)
     int n_tiles;
     double *partials;   // [n_tiles][nq][6]
+};
+
+// What another translation unit needs of an engine (engine.hip).
+struct EngineAccess {
+    int device;
+    int n_cu;
+    hipStream_t stream;
+    hipEvent_t ev0, ev1;  // timing events, used under `mu`
+    std::mutex *mu;       // serialises submissions on `stream`
 };
 
 // launch wrappers (debye_kernels.hip)

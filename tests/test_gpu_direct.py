@@ -1,0 +1,242 @@
+"""GPU parity of the Direct Fourier transform against the CPU oracle
+(oracle/direct_oracle.c), restating the reference's tests:
+
+* gpu_matches_cpu_direct_vacuum (src/gpu.rs:666-700): 4LZT, 20 residue kinds,
+  Miller pmax 4, 200 A box;
+* gpu_matches_cpu_num_molecules (src/gpu.rs:702-751): three identical 2-atom
+  molecules, total and per-molecule self term;
+* direct_solvent_matches_cached_model_for_matching_knobs and
+  fraser_volume_scale_is_not_applied_twice (src/lib.rs:568-660), bitwise;
+* the phase-table setup of src/explicit.rs:480-515 (lattice grid, 4 molecules).
+
+The reference holds its f32 GPU path to 1e-3; the tolerances here are the
+engine's own: FP32 1e-5 and FP64 1e-12 relative per |q| row against the
+extended-precision oracle."""
+from __future__ import annotations
+
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import REFDATA
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp32": 1e-5, "fp64": 1e-12}
+
+
+def _grouped(values, group, n_groups):
+    out = np.zeros(n_groups)
+    cnt = np.zeros(n_groups)
+    for v, g in zip(values, group):
+        out[g] += v
+        cnt[g] += 1
+    return out / cnt
+
+
+def _rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+
+def _lysozyme(engine):
+    import pripps_amd
+    p = pripps_amd.Pripps(os.path.join(REFDATA, "poly_amino_acid.yaml"), engine=engine)
+    xyz = os.path.join(REFDATA, "4lzt.xyz")
+    st = pripps_amd.structure_from_file(xyz, p.formfactors.kinds())
+    return p, xyz, st
+
+
+def _tables(p, st, qn):
+    used = sorted(set(st.ids.tolist()))
+    remap = {k: r for r, k in enumerate(used)}
+    kind = np.array([remap[i] for i in st.ids], dtype=np.int32)
+    return used, kind, p.formfactors.table(used, qn)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_gpu_matches_cpu_direct_vacuum(engine, oracle, precision):
+    from pripps_amd.direct import QGrid
+    p, xyz, st = _lysozyme(engine)
+    box = (200.0, 200.0, 200.0)
+    res = p.intensity(xyz, model="direct", pmax=4, box_sides=200.0, precision=precision)
+    idx, qn, group, gq = QGrid.miller(4).groups(box)
+    assert np.array_equal(res.q, gq)
+    _, kind, ffa = _tables(p, st, qn)
+    rows = oracle.direct_rows(st.pos, kind, box, idx, ffa, mode=oracle.DIRECT_EXTENDED)
+    ref = _grouped(rows[:, 0] ** 2 + rows[:, 1] ** 2, group, len(gq))
+    err = _rel(res.total, ref)
+    print(f"direct vacuum 4LZT pmax 4 {precision}: max rel err {err:.3e}")
+    assert err <= TOL[precision]
+    assert res.atoms is None and res.p is None
+    # the reference's own order of arithmetic agrees to its rounding
+    rows_ref = oracle.direct_rows(st.pos, kind, box, idx, ffa, mode=oracle.DIRECT_REFERENCE)
+    assert _rel(_grouped(rows_ref[:, 0] ** 2 + rows_ref[:, 1] ** 2, group, len(gq)), ref) < 1e-12
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_gpu_matches_cpu_num_molecules(engine, oracle, tmp_path, precision):
+    import pripps_amd
+    from pripps_amd.direct import QGrid
+    yaml = tmp_path / "x.yaml"
+    yaml.write_text("X: { formfactor: !Constant 1.0 }\n")
+    xyz = tmp_path / "nmol.xyz"
+    xyz.write_text("6\nb\nX 0 0 0\nX 0 0 4\nX 30 10 0\nX 30 10 4\nX 70 50 0\nX 70 50 4\n")
+    p = pripps_amd.Pripps(str(yaml), engine=engine)
+    res = p.intensity(str(xyz), model="direct", pmax=6, box_sides=100.0, num_molecules=3,
+                      precision=precision)
+    box = (100.0,) * 3
+    idx, qn, group, gq = QGrid.miller(6).groups(box)
+    pos = np.array([[0, 0, 0], [0, 0, 4], [30, 10, 0], [30, 10, 4], [70, 50, 0], [70, 50, 4.0]])
+    rows = oracle.direct_rows(pos, np.zeros(6, np.int32), box, idx, np.ones((1, len(idx))),
+                              num_molecules=3)
+    total = _grouped(rows[:, 0] ** 2 + rows[:, 1] ** 2, group, len(gq))
+    self_term = _grouped(rows[:, 6], group, len(gq))
+    et, es = _rel(res.total, total), _rel(res.p * 3.0, self_term)
+    print(f"direct num_molecules {precision}: total {et:.3e} self {es:.3e}")
+    assert et <= TOL[precision] and es <= TOL[precision]
+    assert np.allclose(res.s_eff, res.total / (res.p * 3.0), rtol=1e-15)
+
+
+def _solvent_model(engine, precision, **kw):
+    p, xyz, st = _lysozyme(engine)
+    rng = np.random.default_rng(17)
+    w = np.where(rng.uniform(size=len(st.ids)) < 0.4, rng.uniform(size=len(st.ids)), 0.0)
+    m = p.direct_model(xyz, pmax=3, side_length=200.0, excluded="fraser", hydration="weights",
+                       hydration_weights=w, precision=precision, **kw)
+    return p, xyz, st, w, m
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_direct_solvent_model_matches_oracle(engine, oracle, precision):
+    from pripps_amd.direct import QGrid
+    p, xyz, st, w, m = _solvent_model(engine, precision)
+    box = (200.0,) * 3
+    idx, qn, group, gq = QGrid.miller(3).groups(box)
+    inp = p.prepare_input(st.pos, st.ids, qn, excluded="fraser", hydration="weights",
+                          hydration_weights=w)
+    rows = oracle.direct_rows(st.pos, inp.kind, box, idx, inp.atom_ff, inp.excluded_ff,
+                              inp.water_ff, inp.weights)
+    A = rows[:, 0] + 1j * rows[:, 1]
+    Ex = rows[:, 2] + 1j * rows[:, 3]
+    H = rows[:, 4] + 1j * rows[:, 5]
+    worst = 0.0
+    for vs, cd in ((1.0, 0.03), (0.95, -2.0), (1.12, 4.0), (1.7, 0.0)):
+        got = m.intensity(vs, cd)
+        ref = _grouped(np.abs(A - vs * Ex + cd * H) ** 2, group, len(gq))
+        worst = max(worst, _rel(got.total, ref))
+        for name, amp in (("atoms", A), ("excluded", Ex), ("hydration", H)):
+            worst = max(worst, _rel(getattr(got, name), _grouped(np.abs(amp) ** 2, group, len(gq))))
+    print(f"direct solvent 4LZT pmax 3 {precision}: max rel err {worst:.3e}")
+    assert worst <= TOL[precision]
+    # the six profiles combine to the total (Debye combine)
+    P = m.profiles()
+    for vs, cd in ((1.0, 0.03), (1.12, 4.0)):
+        comb = P[0] + vs * vs * P[1] + cd * cd * P[2] - 2 * vs * P[3] + 2 * cd * P[4] - 2 * vs * cd * P[5]
+        assert _rel(comb, m.intensity(vs, cd).total) < 1e-9
+
+
+def test_direct_one_shot_equals_cached_model(engine):
+    """src/lib.rs:568-596 and 598-660 (the Direct halves), bitwise."""
+    p, xyz, st, w, m = _solvent_model(engine, "fp32")
+    one = p.intensity(xyz, model="direct", pmax=3, box_sides=200.0, excluded="fraser",
+                      hydration="weights", hydration_weights=w, volume_scale=1.0,
+                      contrast_density=0.03)
+    cached = m.intensity(1.0, 0.03)
+    for name in ("q", "total", "atoms", "excluded", "hydration"):
+        assert np.array_equal(getattr(one, name), getattr(cached, name)), name
+    unit = p.direct_model(xyz, pmax=3, side_length=200.0, excluded="fraser")
+    scaled = p.intensity(xyz, model="direct", pmax=3, box_sides=200.0, excluded="fraser",
+                         volume_scale=1.7)
+    assert np.array_equal(scaled.total, unit.intensity(1.7, 0.0).total)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_lattice_grid_molecules(engine, oracle, tmp_path, precision):
+    import pripps_amd
+    from pripps_amd.direct import QGrid
+    yaml = tmp_path / "xy.yaml"
+    yaml.write_text("X: { formfactor: !Constant 1.0 }\nY: { formfactor: !Constant 0.5 }\n")
+    L = 37.0
+    pos = np.array([[math.fmod(7.3 * t, L) - 5.0, math.fmod(3.1 * t * t, L),
+                     math.fmod(11.7 * t + 2.0, L)] for t in map(float, range(12))])
+    ids = np.array([i % 2 for i in range(12)])
+    p = pripps_amd.Pripps(str(yaml), engine=engine)
+    grid = QGrid.lattice([L] * 3, 0.9, 40)
+    m = p.direct_model_from_arrays(pos, ids, box_sides=L, grid=grid, num_molecules=4,
+                                   precision=precision)
+    idx, qn, group, gq = grid.groups((L,) * 3)
+    assert np.array_equal(m.q_values, gq)
+    ffa = np.array([np.full(len(idx), 1.0), np.full(len(idx), 0.5)])
+    rows = oracle.direct_rows(pos, ids.astype(np.int32), (L,) * 3, idx, ffa, num_molecules=4)
+    et = _rel(m.intensity(1.0, 0.0).total, _grouped(rows[:, 0] ** 2 + rows[:, 1] ** 2, group, len(gq)))
+    es = _rel(m.self_term(), _grouped(rows[:, 6], group, len(gq)))
+    amp = m.amplitudes()
+    ea = float(np.max(np.abs(amp[:, :2] - rows[:, :2]))) / float(np.max(np.abs(rows[:, :2])))
+    print(f"direct lattice L=37 4 molecules {precision}: total {et:.3e} self {es:.3e} amp {ea:.3e}")
+    assert max(et, es, ea) <= TOL[precision]
+
+
+def test_direct_fit_recovers_knobs(engine):
+    p, xyz, st, w, m = _solvent_model(engine, "fp64")
+    q = m.q_values
+    target = m.intensity(1.2, 0.05).total
+    sigma = 0.01 * target
+    fit = m.fit(q, target, sigma, method="grid")
+    assert abs(fit.volume_scale - 1.2) < 1e-9 and abs(fit.contrast_density - 0.05) < 1e-9
+    assert fit.chi2 < 1e-10 and abs(fit.scale - 1.0) < 1e-9
+    nm = m.fit(q, target, sigma)
+    assert abs(nm.volume_scale - 1.2) < 1e-3 and abs(nm.contrast_density - 0.05) < 1e-3
+
+
+def test_direct_edges_and_determinism(engine, oracle):
+    import pripps_amd
+    p, xyz, st = _lysozyme(engine)
+    # no q-vectors: an empty curve
+    empty = p.intensity(xyz, model="direct", pmax=0, box_sides=200.0)
+    assert len(empty.q) == 0 and len(empty.total) == 0
+    # no atoms: zero intensity on every row
+    m0 = p.direct_model_from_arrays(np.zeros((0, 3)), np.zeros(0, dtype=int), pmax=2,
+                                    box_sides=80.0)
+    i0 = m0.intensity(1.0, 0.0)
+    assert len(i0.q) > 0 and np.all(i0.total == 0.0)
+    # two builds give bitwise the same amplitudes; an orthorhombic box works
+    a = p.direct_model(xyz, pmax=4, box_sides=(180.0, 200.0, 220.0)).amplitudes()
+    b = p.direct_model(xyz, pmax=4, box_sides=(180.0, 200.0, 220.0)).amplitudes()
+    assert np.array_equal(a, b)
+    # coordinates far outside the box reduce exactly (fractional coordinates)
+    far = st.pos + np.array([180.0, -2 * 200.0, 3 * 220.0])
+    c = p.direct_model_from_arrays(far, st.ids, pmax=4, box_sides=(180.0, 200.0, 220.0),
+                                   precision="fp64").amplitudes()
+    d = p.direct_model_from_arrays(st.pos, st.ids, pmax=4, box_sides=(180.0, 200.0, 220.0),
+                                   precision="fp64").amplitudes()
+    assert np.max(np.abs(c[:, :2] - d[:, :2])) <= 1e-12 * np.max(np.abs(d[:, :2]))
+    with pytest.raises(pripps_amd.UsageError):
+        p.direct_model(xyz, pmax=2, box_sides=(0.0, 1.0, 1.0))
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_uneven_molecule_chunks(engine, oracle, tmp_path, precision):
+    """Molecules are chunks of max(n / N, 1) atoms, the last one shorter when N
+    does not divide n (src/explicit.rs:222-236); mixed kinds inside chunks."""
+    import pripps_amd
+    from pripps_amd.direct import QGrid
+    yaml = tmp_path / "xy.yaml"
+    yaml.write_text("X: { formfactor: !Constant 1.0 }\nY: { formfactor: !Constant 0.5 }\n")
+    p = pripps_amd.Pripps(str(yaml), engine=engine)
+    rng = np.random.default_rng(23)
+    for n, nmol in ((7, 3), (1000, 7), (3, 5)):
+        pos = rng.uniform(-40, 40, size=(n, 3))
+        ids = rng.integers(0, 2, size=n)
+        m = p.direct_model_from_arrays(pos, ids, pmax=5, box_sides=(60.0, 70.0, 80.0),
+                                       num_molecules=nmol, precision=precision)
+        idx, qn, group, gq = QGrid.miller(5).groups((60.0, 70.0, 80.0))
+        ffa = np.array([np.full(len(idx), 1.0), np.full(len(idx), 0.5)])
+        rows = oracle.direct_rows(pos, ids.astype(np.int32), (60.0, 70.0, 80.0), idx, ffa,
+                                  num_molecules=nmol)
+        et = _rel(m.intensity(1.0, 0.0).total, _grouped(rows[:, 0] ** 2 + rows[:, 1] ** 2, group, len(gq)))
+        es = _rel(m.self_term(), _grouped(rows[:, 6], group, len(gq)))
+        print(f"direct chunks n={n} N={nmol} {precision}: total {et:.3e} self {es:.3e}")
+        assert max(et, es) <= TOL[precision]

@@ -20,10 +20,12 @@ from pripps_amd.structure import parse_xyz_text, read_structure, resolve_ids, wr
 
 
 def test_library_exports_every_declared_symbol():
-    header = open(os.path.join(REPO, "include", "pripps_debye.h")).read()
-    header = re.sub(r"/\*.*?\*/", "", header, flags=re.S)
-    declared = set(re.findall(r"\b(pripps_[a-z_0-9]+)\s*\(", header))
-    assert len(declared) >= 19
+    declared = set()
+    for name in ("pripps_debye.h", "pripps_direct.h"):
+        header = open(os.path.join(REPO, "include", name)).read()
+        header = re.sub(r"/\*.*?\*/", "", header, flags=re.S)
+        declared |= set(re.findall(r"\b(pripps_[a-z_0-9]+)\s*\(", header))
+    assert len(declared) >= 29
     lib = ctypes.CDLL(os.path.join(REPO, "pripps_amd", "libpripps_debye.so"))
     missing = [s for s in sorted(declared) if not hasattr(lib, s)]
     assert not missing, missing
