@@ -75,7 +75,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--workload", default="solvent10k",
-                    choices=("solvent10k", "vacuum1k", "helix60k", "batch512", "served"))
+                    choices=("solvent10k", "vacuum1k", "helix60k", "batch512", "served", "direct"))
+    ap.add_argument("--cpu-vectors", type=int, default=256,
+                    help="direct: q-vectors of the oracle / CPU baseline sample")
     ap.add_argument("--count", type=int, default=512, help="batch512: structures")
     ap.add_argument("--plain-sample", type=int, default=32,
                     help="batch512: structures timed on the sequential plain FP64 path")
@@ -495,8 +497,95 @@ def bench_served(args):
     print(json.dumps({"workload": SERVED_WORKLOAD, "rates": results}), flush=True)
 
 
+def direct_box_workload(n_mol=256, n_atoms=1000, box=400.0, seed=6):
+    """n_mol copies of one seeded globule (same element sequence), each rotated
+    at random and placed on a jittered cubic lattice in a periodic box."""
+    import numpy as np
+    from scipy.spatial.transform import Rotation
+    from pripps_amd.synth import globule, radius_for
+    names, pos, _ = globule(n_atoms, radius_for(n_atoms), seed, solvent=False)
+    pos = pos - pos.mean(axis=0)
+    rng = np.random.default_rng(seed)
+    side = int(math.ceil(n_mol ** (1.0 / 3.0)))
+    cell = box / side
+    sites = [(i, j, k) for i in range(side) for j in range(side) for k in range(side)][:n_mol]
+    rot = Rotation.random(n_mol, random_state=seed).as_matrix()
+    parts = []
+    for m, (i, j, k) in enumerate(sites):
+        centre = (np.array([i, j, k]) + 0.5) * cell + rng.uniform(-0.1, 0.1, 3) * cell
+        parts.append(pos @ rot[m].T + centre)
+    return names * n_mol, np.vstack(parts)
+
+
+def bench_direct(args):
+    """Direct Fourier transform of a periodic box of identical molecules on a
+    lattice q-grid, with the per-molecule self term (effective structure
+    factor): FP32 and FP64 against the oracle, and the oracle's reference-order
+    loop (libm cos/sin, FP64) on a sample of q-vectors as the CPU baseline."""
+    import numpy as np
+    from pripps_amd.direct import QGrid
+    O = _oracle()
+    eng, ff = _engine_and_ff()
+    n_mol, box = 256, 400.0
+    t = time.perf_counter()
+    names, pos = direct_box_workload(n_mol=n_mol, box=box)
+    ids = ff.names_to_ids(names)
+    grid = QGrid.lattice([box] * 3, 0.5, 500)
+    idx, qn, group, gq = grid.groups((box,) * 3)
+    terms = float(len(pos)) * len(idx)
+    log(f"direct: {len(pos)} atoms, {len(idx)} q-vectors in {len(gq)} bins, {terms:.3e} atom-vector "
+        f"terms (setup {time.perf_counter() - t:.1f} s)")
+    out = {"workload": f"direct: {n_mol} x 1,000-atom globules (rotated, jittered lattice) in a "
+                       f"{box:.0f} A periodic box, lattice q-grid q_max 0.5 max 500 vectors per bin, "
+                       "per-molecule self term",
+           "n_atoms": len(pos), "n_vectors": len(idx), "n_bins": len(gq), "atom_vector_terms": terms}
+    res = {}
+    for prec in ("fp64", "fp32"):
+        m = ff.direct_model_from_arrays(pos, ids, box_sides=box, grid=grid, num_molecules=n_mol,
+                                        precision=prec)
+        m._native.recompute(1)
+        t0 = time.perf_counter()
+        kms = m._native.recompute(args.steps)
+        wall = (time.perf_counter() - t0) * 1e3 / max(args.steps, 1)
+        res[prec] = (m.amplitudes(), m.intensity(1.0, 0.0).total, m.self_term())
+        out[prec] = {"kernel_ms": kms, "ms_per_step": wall, "terms_per_s": terms / (kms * 1e-3),
+                     "work_items": int(m.stats().work_items)}
+        log(f"direct {prec}: {out[prec]}")
+        del m
+    a32, a64 = res["fp32"][0], res["fp64"][0]
+    out["fp32_vs_fp64"] = {
+        "total_max_rel": float(np.max(np.abs(res["fp32"][1] - res["fp64"][1]) / res["fp64"][1])),
+        "self_max_rel": float(np.max(np.abs(res["fp32"][2] - res["fp64"][2]) / res["fp64"][2])),
+        "amplitude_max_abs_over_max": float(np.max(np.abs(a32[:, :2] - a64[:, :2])) / np.max(np.abs(a64[:, :2])))}
+    # oracle (extended) and CPU baseline (reference order) on evenly spaced vectors
+    k = min(args.cpu_vectors, len(idx))
+    sel = np.unique(np.linspace(0, len(idx) - 1, k).round().astype(int))
+    inp = ff.prepare_input(pos, ids, qn)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    tc = time.perf_counter()
+    ref_order = O.direct_rows(pos, inp.kind, (box,) * 3, idx[sel], inp.atom_ff[:, sel],
+                              num_molecules=n_mol, mode=O.DIRECT_REFERENCE)
+    tcpu = time.perf_counter() - tc
+    ext = O.direct_rows(pos, inp.kind, (box,) * 3, idx[sel], inp.atom_ff[:, sel],
+                        num_molecules=n_mol, mode=O.DIRECT_EXTENDED)
+    i_ext = ext[:, 0] ** 2 + ext[:, 1] ** 2
+    for prec in ("fp32", "fp64"):
+        a = res[prec][0][sel]
+        out[prec]["vs_oracle_sampled"] = {
+            "intensity_max_rel": float(np.max(np.abs(a[:, 0] ** 2 + a[:, 1] ** 2 - i_ext) / i_ext)),
+            "self_max_rel": float(np.max(np.abs(a[:, 6] - ext[:, 6]) / ext[:, 6]))}
+    out["cpu_baseline"] = {"value": len(pos) * len(sel) / tcpu, "unit": "atom-vector terms/s",
+                           "cores": threads, "kind": "port",
+                           "sample": f"oracle reference-order FP64 loop (libm cos/sin) over all "
+                                     f"{len(pos)} atoms for {len(sel)} evenly spaced q-vectors, "
+                                     f"{tcpu:.1f} s"}
+    out["speedup_fp32_vs_cpu"] = out["fp32"]["terms_per_s"] / out["cpu_baseline"]["value"]
+    log(f"direct oracle/cpu: {out['cpu_baseline']}; fp32 {out['fp32']['vs_oracle_sampled']}")
+    print(json.dumps(out), flush=True)
+
+
 WORKLOADS = {"vacuum1k": bench_vacuum1k, "helix60k": bench_helix60k, "batch512": bench_batch512,
-             "served": bench_served}
+             "served": bench_served, "direct": bench_direct}
 
 
 if __name__ == "__main__":

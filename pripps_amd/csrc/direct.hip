@@ -132,18 +132,23 @@ __device__ __forceinline__ void cis_rev(double nx, double ny, double nz, const A
     sincospi(2.0 * f, &s, &c);
 }
 
-// Sums over atoms [jb, je) of one class: C = sum cos, S = sum sin and, for
-// hydrated classes, Cw = sum w cos, Sw = sum w sin.  Two interleaved chains
-// per sum keep consecutive adds independent.
+// Sums over atoms [jb, je) of one class (je > jb): C = sum cos, S = sum sin
+// and, for hydrated classes, Cw = sum w cos, Sw = sum w sin.  Two atoms per
+// step with two chains per sum keep consecutive adds independent, and the
+// next two atoms' scalar loads are issued before the current two are used
+// (every index is clamped to je - 1, so no load leaves the run).
 template <typename T, bool WET>
 __device__ __forceinline__ void class_run(const PRIPPS_CONST Atom<T> *ca, int jb, int je, T nx,
                                           T ny, T nz, T &C, T &S, T &Cw, T &Sw) {
     T c0 = T(0), s0 = T(0), c1 = T(0), s1 = T(0);
     T cw0 = T(0), sw0 = T(0), cw1 = T(0), sw1 = T(0);
+    const int last = je - 1;
+    Atom<T> a = load_frac(ca, jb);
+    Atom<T> b = load_frac(ca, min(jb + 1, last));
     int j = jb;
     for (; j + 1 < je; j += 2) {
-        const Atom<T> a = load_frac(ca, j);
-        const Atom<T> b = load_frac(ca, j + 1);
+        const Atom<T> na = load_frac(ca, min(j + 2, last));
+        const Atom<T> nb = load_frac(ca, min(j + 3, last));
         T ca_, sa_, cb_, sb_;
         cis_rev(nx, ny, nz, a, ca_, sa_);
         cis_rev(nx, ny, nz, b, cb_, sb_);
@@ -157,9 +162,10 @@ __device__ __forceinline__ void class_run(const PRIPPS_CONST Atom<T> *ca, int jb
             cw1 = fma(b.w, cb_, cw1);
             sw1 = fma(b.w, sb_, sw1);
         }
+        a = na;
+        b = nb;
     }
-    if (j < je) {
-        const Atom<T> a = load_frac(ca, j);
+    if (j < je) {  // odd count: `a` holds atom j
         T ca_, sa_;
         cis_rev(nx, ny, nz, a, ca_, sa_);
         c0 += ca_;
@@ -259,40 +265,47 @@ __global__ void __launch_bounds__(256) direct_finalize_kernel(FinalizeDirectArgs
     for (int t = 0; t < kDirVals; ++t) a.amp[(int64_t)v * kDirVals + t] = acc[t];
 }
 
-// Group means, in the reference's operation order: per vector
+// Group means, in the reference's operation order per vector:
 // |A - vs E + cd H|^2, |A|^2, |E|^2, |H|^2 (norm_sqr = re re + im im), summed
-// over the group's vectors in ascending order and divided by the count.  The
-// cross means Re A E*, Re A H*, Re E H* complete the six profiles in which
-// the total is the Debye combine: |A|^2 + vs^2 |E|^2 + cd^2 |H|^2
-// - 2 vs Re A E* + 2 cd Re A H* - 2 vs cd Re E H*.
-__global__ void __launch_bounds__(256) direct_group_kernel(GroupArgs a) {
+// over the group's vectors and divided by the count.  The cross means
+// Re A E*, Re A H*, Re E H* complete the six profiles in which the total is
+// the Debye combine: |A|^2 + vs^2 |E|^2 + cd^2 |H|^2 - 2 vs Re A E*
+// + 2 cd Re A H* - 2 vs cd Re E H*.  One block per group: thread t takes the
+// group's vectors t, t + 256, ... in order, and a fixed LDS tree combines the
+// threads, so the result depends on the group size only.
+constexpr int kGroupThreads = 256;
+
+__global__ void __launch_bounds__(kGroupThreads) direct_group_kernel(GroupArgs a) {
 #pragma clang fp contract(off)
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= a.G) return;
-    double t = 0, at = 0, ex = 0, hy = 0, sf = 0, ae = 0, ah = 0, eh = 0;
+    __shared__ double red[8][kGroupThreads];
+    const int g = blockIdx.x;
+    const int tid = threadIdx.x;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     const int b = a.group_ptr[g], e = a.group_ptr[g + 1];
-    for (int p = b; p < e; ++p) {
+    for (int p = b + tid; p < e; p += kGroupThreads) {
         const double *r = a.amp + (int64_t)a.group_vec[p] * kDirVals;
         const double re = (r[0] - a.vs * r[2]) + a.cd * r[4];
         const double im = (r[1] - a.vs * r[3]) + a.cd * r[5];
-        t += re * re + im * im;
-        at += r[0] * r[0] + r[1] * r[1];
-        ex += r[2] * r[2] + r[3] * r[3];
-        hy += r[4] * r[4] + r[5] * r[5];
-        sf += r[6];
-        ae += r[0] * r[2] + r[1] * r[3];
-        ah += r[0] * r[4] + r[1] * r[5];
-        eh += r[2] * r[4] + r[3] * r[5];
+        acc[0] += re * re + im * im;
+        acc[1] += r[0] * r[0] + r[1] * r[1];
+        acc[2] += r[2] * r[2] + r[3] * r[3];
+        acc[3] += r[4] * r[4] + r[5] * r[5];
+        acc[4] += r[6];
+        acc[5] += r[0] * r[2] + r[1] * r[3];
+        acc[6] += r[0] * r[4] + r[1] * r[5];
+        acc[7] += r[2] * r[4] + r[3] * r[5];
     }
-    const double n = double(e - b);
-    a.out[0 * a.G + g] = t / n;
-    a.out[1 * a.G + g] = at / n;
-    a.out[2 * a.G + g] = ex / n;
-    a.out[3 * a.G + g] = hy / n;
-    a.out[4 * a.G + g] = sf / n;
-    a.out[5 * a.G + g] = ae / n;
-    a.out[6 * a.G + g] = ah / n;
-    a.out[7 * a.G + g] = eh / n;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) red[t][tid] = acc[t];
+    __syncthreads();
+    for (int half = kGroupThreads / 2; half > 0; half /= 2) {
+        if (tid < half) {
+#pragma unroll
+            for (int t = 0; t < 8; ++t) red[t][tid] += red[t][tid + half];
+        }
+        __syncthreads();
+    }
+    if (tid < 8) a.out[tid * a.G + g] = red[tid][0] / double(e - b);
 }
 
 __global__ void direct_fill_int_kernel(int *p, int v) { *p = v; }
@@ -486,7 +499,7 @@ extern "C" int pripps_direct_model_build(pripps_engine *engine, const pripps_deb
     int64_t span = 0;
     std::vector<DirectItem> items;
     if (N > 0 && V > 0) {
-        const int64_t target_items = 16LL * acc.n_cu;
+        const int64_t target_items = 64LL * acc.n_cu;  // several items per wave slot
         span = std::max<int64_t>(256, (N * n_tiles + target_items - 1) / target_items);
         if (self_mode) span = std::max<int64_t>(1, (span + M - 1) / M) * M;
         span = std::min<int64_t>(span, N);
@@ -660,7 +673,7 @@ static int run_groups(const pripps_direct_model *m, double vs, double cd, std::v
     GroupArgs g = m->gargs;
     g.vs = vs;
     g.cd = cd;
-    hipLaunchKernelGGL(direct_group_kernel, dim3((m->G + 255) / 256), dim3(256), 0, m->acc.stream, g);
+    hipLaunchKernelGGL(direct_group_kernel, dim3(m->G), dim3(kGroupThreads), 0, m->acc.stream, g);
     DCHK(hipGetLastError());
     DCHK(hipMemcpyAsync(h.data(), m->d_out, h.size() * sizeof(double), hipMemcpyDeviceToHost,
                         m->acc.stream));
