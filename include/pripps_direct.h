@@ -103,6 +103,24 @@ int pripps_direct_model_recompute(pripps_direct_model *model, int32_t repeats,
 
 int pripps_direct_model_free(pripps_direct_model *model);
 
+/* Weighted average of I(q) over trajectory frames for the Direct scheme
+ * (reference trajectory_average, src/trajectory.rs:167-251, with XtcIterator's
+ * stride and weights, src/trajectory.rs:29-135).  `topology` gives kinds,
+ * tables and weights (its x/y/z are not read); coords is
+ * [n_frames][n_atoms][3] in Angstrom; frames 0, stride, 2 stride, ... are
+ * used, with frame_weights (NULL: 1 each) indexed by the original frame.  The
+ * box is fixed (the reference accumulates against the first frame's q grid).
+ * out_total[n_groups] receives the weighted mean of group means of
+ * |A - vs E + cd H|^2 (vs, cd baked in per frame, as the one-shot path does);
+ * out_self[n_groups] (or NULL) the weighted mean self term, which needs
+ * num_molecules > 0.  Positions are re-staged per frame into the resident
+ * model; amplitudes and accumulators never leave the device. */
+int pripps_direct_frames_average(pripps_engine *engine, const pripps_debye_input *topology,
+                                 const pripps_direct_grid *grid, int32_t n_frames,
+                                 const double *coords, const double *frame_weights,
+                                 int32_t stride, int32_t precision, double volume_scale,
+                                 double contrast_density, double *out_total, double *out_self);
+
 #ifdef __cplusplus
 }
 #endif

@@ -430,6 +430,75 @@ class Pripps:
                               num_molecules=num_molecules, precision=precision)
         return DirectModel(native, self.engine, len(idx))
 
+    def direct_trajectory_average(self, structure_file, frames, *, box_sides, pmax: int = 6,
+                                  grid=None, num_molecules=None, weights=None, stride: int = 1,
+                                  units: str = "angstrom", excluded=None, hydration=None,
+                                  probe: float = 1.8, spacing: float = 3.10516,
+                                  thickness: float = 3.0, excluded_spacing: float = 1.0,
+                                  bulk_electron_density: float = 0.334,
+                                  volume_scale: float = 1.0, contrast_density: float = 0.03,
+                                  hydration_weights=None, precision: str = "fp32") -> Intensity:
+        """Weighted average of the Direct I(q) over trajectory frames: the
+        reference's ``trajectory_average`` (src/trajectory.rs:167-251) with the
+        Direct scheme, a fixed box and ``XtcIterator``'s stride / weights
+        (src/trajectory.rs:29-135).  ``frames`` is [n_frames][n_atoms][3] in
+        ``units`` ("angstrom" or "nm", the XTC factor 10); ``weights`` an array
+        or a weights file indexed by the original frame.  With
+        ``num_molecules`` the result carries P(q) and S_eff(q) formed from the
+        averaged self term (src/lib.rs:511-526)."""
+        from .direct import QGrid, check_box, frames_average, parse_box
+        if units not in ("angstrom", "nm"):
+            raise ValueError(f"unknown units: {units} (expected `angstrom` or `nm`)")
+        if int(stride) < 1:
+            raise ValueError("`stride` must be >= 1")
+        st = structure_from_file(structure_file, self.formfactors.kinds())
+        ids = st.ids
+        f = np.asarray(frames, dtype=np.float64)
+        if f.ndim != 3 or f.shape[1:] != (len(ids), 3):
+            raise ValidationError(f"number of atoms in structure ({len(ids)}) does not match "
+                                  f"trajectory frames of shape {list(f.shape)}")
+        if units == "nm":
+            f = f * 10.0
+        if isinstance(weights, (str, os.PathLike)):
+            weights = read_weights(weights)
+        ex, hy = excluded_name(excluded), hydration_name(hydration)
+        if num_molecules is not None:
+            n = int(num_molecules)
+            if n == 0:
+                raise UsageError("--num-molecules must be at least 1")
+            if ex != "none" or hy != "none":
+                raise UsageError("--num-molecules supports vacuum runs only (no "
+                                 "excluded-volume/hydration model)")
+            if len(ids) % n != 0:
+                raise ValidationError(f"box atom count ({len(ids)}) is not a multiple of "
+                                      f"--num-molecules ({n})")
+        box = parse_box(box_sides)
+        check_box(box)
+        grid = QGrid.miller(pmax) if grid is None else grid
+        idx, qn, group, gq = grid.groups(box)
+        pos0 = f[0] if len(f) else np.zeros((len(ids), 3))
+        inp = self.prepare_input(pos0, ids, qn, excluded=excluded, hydration=hydration,
+                                 hydration_weights=hydration_weights,
+                                 bulk_electron_density=bulk_electron_density, probe=probe,
+                                 spacing=spacing, thickness=thickness,
+                                 excluded_spacing=excluded_spacing)
+        if len(inp.x) != len(ids):
+            raise UsageError("grid solvent models add dummies per frame; trajectories take "
+                             "colocated solvent models only")
+        nmol = 0 if num_molecules is None else int(num_molecules)
+        vs, cd = (volume_scale, contrast_density) if (ex != "none" or hy != "none") else (1.0, 0.0)
+        total, self_term = frames_average(self.engine, inp, box, idx, group, gq, f,
+                                          weights=weights, stride=stride, num_molecules=nmol,
+                                          precision=precision, volume_scale=vs,
+                                          contrast_density=cd)
+        res = Intensity(gq.copy(), total)
+        if self_term is not None:
+            res.p = self_term / float(nmol)
+            with np.errstate(divide="ignore", invalid="ignore"):
+                res.s_eff = np.where(self_term > 0.0,
+                                     total / np.where(self_term > 0, self_term, 1.0), np.nan)
+        return res
+
     def intensity(self, structure_file, *, model=None, xtc=None, stride: int = 1, weights=None,
                   box_sides=None, excluded=None, hydration=None, qmin: float = 0.0,
                   qmax: float = 0.5, nq=None, l_max=None, pmax: int = 6, probe: float = 1.8,

@@ -98,7 +98,7 @@ EXPORTS = (
     "pripps_direct_model_build", "pripps_direct_model_n_groups", "pripps_direct_model_q_values",
     "pripps_direct_model_intensity", "pripps_direct_model_self_term",
     "pripps_direct_model_profiles", "pripps_direct_model_amplitudes", "pripps_direct_model_stats",
-    "pripps_direct_model_recompute", "pripps_direct_model_free",
+    "pripps_direct_model_recompute", "pripps_direct_model_free", "pripps_direct_frames_average",
 )
 
 _lib = None
@@ -174,6 +174,10 @@ def load():
         L.pripps_direct_model_recompute.argtypes = [vp, ctypes.c_int32,
                                                     ctypes.POINTER(ctypes.c_double)]
         L.pripps_direct_model_free.argtypes = [vp]
+        L.pripps_direct_frames_average.argtypes = [vp, ctypes.POINTER(DebyeInput),
+                                                   ctypes.POINTER(DirectGrid), ctypes.c_int32,
+                                                   _D, _D, ctypes.c_int32, ctypes.c_int32,
+                                                   ctypes.c_double, ctypes.c_double, _D, _D]
         for name in EXPORTS:
             if name not in ("pripps_last_error", "pripps_version"):
                 getattr(L, name).restype = ctypes.c_int

@@ -48,6 +48,20 @@ __device__ __forceinline__ const PRIPPS_CONST T *as_const(const T *p) {
     return (const PRIPPS_CONST T *)(p);
 }
 
+// A copy of x in a vector register.  On gfx950 a VALU instruction that
+// reads a scalar register issues at half the rate of one whose operands are
+// all vector registers (measured: 0.25 vs 0.47 wave instructions per
+// SIMD-cycle for v_add_f32 / v_fmac_f32), so wave-uniform values that feed
+// an inner loop are moved once into a VGPR.  The register move is opaque to
+// the compiler, which would otherwise fold the scalar operand back in.
+__device__ __forceinline__ float to_vgpr(float x) {
+    float v;
+    asm("v_mov_b32 %0, %1" : "=v"(v) : "v"(x));
+    return v;
+}
+
+__device__ __forceinline__ double to_vgpr(double x) { return x; }
+
 template <typename T>
 __device__ __forceinline__ Atom<T> load_atom(const PRIPPS_CONST Atom<T> *p, int j) {
     Atom<T> a;
@@ -345,13 +359,14 @@ __device__ __forceinline__ void pair_block_f32x2(const Atom<float> *__restrict__
         const Seed32 zb = seed_f32(bx * bx + by * by + bz * bz, q0p, q0lo, hp, hlo, tb, SMALL_H);
         float s1 = za.s, d1 = za.d, s2 = zb.s, d2 = zb.d;
         const float l1 = za.lam, l2 = zb.lam;
+        const float wa = WET ? to_vgpr(a.w) : 0.0f, wb = WET ? to_vgpr(b.w) : 0.0f;
         if constexpr (INTERLEAVE) {
 #pragma unroll
             for (int k = 0; k < Q; ++k) {
                 acc0[k] += s1 + s2;
                 if constexpr (WET) {
-                    accw[k] = __builtin_fmaf(a.w, s1, accw[k]);
-                    accw[k] = __builtin_fmaf(b.w, s2, accw[k]);
+                    accw[k] = __builtin_fmaf(wa, s1, accw[k]);
+                    accw[k] = __builtin_fmaf(wb, s2, accw[k]);
                 }
                 s1 += d1;
                 s2 += d2;
@@ -362,14 +377,14 @@ __device__ __forceinline__ void pair_block_f32x2(const Atom<float> *__restrict__
 #pragma unroll
             for (int k = 0; k < Q; ++k) {
                 acc0[k] += s1;
-                if constexpr (WET) accw[k] = __builtin_fmaf(a.w, s1, accw[k]);
+                if constexpr (WET) accw[k] = __builtin_fmaf(wa, s1, accw[k]);
                 s1 += d1;
                 d1 = __builtin_fmaf(-l1, s1, d1);
             }
 #pragma unroll
             for (int k = 0; k < Q; ++k) {
                 acc0[k] += s2;
-                if constexpr (WET) accw[k] = __builtin_fmaf(b.w, s2, accw[k]);
+                if constexpr (WET) accw[k] = __builtin_fmaf(wb, s2, accw[k]);
                 s2 += d2;
                 d2 = __builtin_fmaf(-l2, s2, d2);
             }
@@ -382,17 +397,23 @@ __device__ __forceinline__ void pair_block_f32x2(const Atom<float> *__restrict__
         const Seed32 za = seed_f32(ax * ax + ay * ay + az * az, q0p, q0lo, hp, hlo, ta, SMALL_H);
         float s1 = za.s, d1 = za.d;
         const float l1 = za.lam;
+        const float wa = WET ? to_vgpr(a.w) : 0.0f;
 #pragma unroll
         for (int k = 0; k < Q; ++k) {
             acc0[k] += s1;
-            if constexpr (WET) accw[k] = __builtin_fmaf(a.w, s1, accw[k]);
+            if constexpr (WET) accw[k] = __builtin_fmaf(wa, s1, accw[k]);
             s1 += d1;
             d1 = __builtin_fmaf(-l1, s1, d1);
         }
     }
 }
 
-template <typename T, int Q, bool UNIFORM, bool WET, bool MASK, int SMALL_H>
+// BYW: the pass of a hydrated chunk in the one-accumulator kernel that sums
+// w_j s into acc0 (acc0 += w_j s, the same arithmetic as the two-array
+// kernel's accw).  Scaling the pair amplitude by w_j instead would round
+// w_j / r once per pair with an error that depends on r alone; on lattice
+// points, where many pairs share r, that error adds up coherently.
+template <typename T, int Q, bool UNIFORM, bool WET, bool MASK, int SMALL_H, bool BYW = false>
 __device__ __forceinline__ void pair_block(const Atom<T> *__restrict__ atoms, int jb, int je,
                                            const Atom<T> &ai, int iidx, T lane_take,
                                            const T (&qv)[Q], const T (&ql)[Q], T q0p, T q0lo,
@@ -407,6 +428,7 @@ __device__ __forceinline__ void pair_block(const Atom<T> *__restrict__ atoms, in
         next = load_atom(ca, min(j + 1, je - 1));  // prefetch the next j atom
         const T dx = ai.x - aj.x, dy = ai.y - aj.y, dz = ai.z - aj.z;
         const T r2 = dx * dx + dy * dy + dz * dz;
+        const T wj = (WET || BYW) ? to_vgpr(aj.w) : T(0);
         // padding lanes (iidx = INT_MAX, lane_take = 0) never contribute
         const T take = MASK ? (j > iidx ? T(1) : T(0)) : lane_take;
         if constexpr (UNIFORM) {
@@ -424,8 +446,9 @@ __device__ __forceinline__ void pair_block(const Atom<T> *__restrict__ atoms, in
             }
 #pragma unroll
             for (int k = 0; k < Q; ++k) {
-                acc0[k] += s;
-                if constexpr (WET) accw[k] += aj.w * s;
+                if constexpr (BYW) acc0[k] += wj * s;
+                else acc0[k] += s;
+                if constexpr (WET) accw[k] += wj * s;
                 s += d;
                 d -= lam * s;
             }
@@ -440,8 +463,9 @@ __device__ __forceinline__ void pair_block(const Atom<T> *__restrict__ atoms, in
                     // phase (q / 2pi) r with q / 2pi split as qv + ql: see seed_f32
                     const float x = __builtin_fmaf(qv[k], r, ql[k] * r);
                     const float s = __builtin_amdgcn_sinf(__builtin_amdgcn_fractf(x)) * amp;
-                    acc0[k] += s;
-                    if constexpr (WET) accw[k] += aj.w * s;
+                    if constexpr (BYW) acc0[k] += wj * s;
+                    else acc0[k] += s;
+                    if constexpr (WET) accw[k] += wj * s;
                 }
             } else {
                 const double rr2 = fmax(r2, 1e-24);
@@ -450,8 +474,9 @@ __device__ __forceinline__ void pair_block(const Atom<T> *__restrict__ atoms, in
 #pragma unroll
                 for (int k = 0; k < Q; ++k) {
                     const double s = sin(qv[k] * r) * amp;
-                    acc0[k] += s;
-                    if constexpr (WET) accw[k] += aj.w * s;
+                    if constexpr (BYW) acc0[k] += wj * s;
+                    else acc0[k] += s;
+                    if constexpr (WET) accw[k] += wj * s;
                 }
             }
         }
@@ -461,7 +486,13 @@ __device__ __forceinline__ void pair_block(const Atom<T> *__restrict__ atoms, in
 // VARIANT (FP32 uniform grids only): 1 paired loads, chains one after the other;
 // 2 paired loads, chains interleaved; 3 one atom per iteration, half-step
 // sine path chosen per pair; 4 one atom per iteration, path fixed per chunk.
-template <typename T, int Q, bool UNIFORM, int VARIANT>
+// ONEACC (FP32 on uniform grids, variants 3 and 4): one accumulator array.
+// A hydrated j chunk runs twice, first for sum s, then for sum w_j s.  Atoms are sorted hydrated-first, so
+// a hydrated j only meets the hydrated i before it and the second pass covers
+// a few percent of the pairs; in exchange the kernel needs about 60 fewer
+// VGPRs and runs three waves per SIMD instead of two, which lets the VALU
+// dual-issue far more often.
+template <typename T, int Q, bool UNIFORM, int VARIANT, bool ONEACC = false>
 __global__ void __launch_bounds__(256) debye_tiled_kernel(TiledArgs args) {
     constexpr bool INTERLEAVE = VARIANT == 2;
     constexpr int M = Split<Q>::M;
@@ -499,6 +530,11 @@ __global__ void __launch_bounds__(256) debye_tiled_kernel(TiledArgs args) {
                 const double hd = sd.dq * (0.5 * kInv2Pi);
                 hp = float(hd);
                 hlo = float(hd - double(hp));
+                // item-uniform: used by every pair's seed
+                q0p = to_vgpr(q0p);
+                q0lo = to_vgpr(q0lo);
+                hp = to_vgpr(hp);
+                hlo = to_vgpr(hlo);
             } else {
                 q0p = qs;
                 hp = 0.5 * sd.dq;
@@ -541,6 +577,37 @@ __global__ void __launch_bounds__(256) debye_tiled_kernel(TiledArgs args) {
             const bool wet = sd.cls_wet[c] != 0;
             const bool mask = j <= it.i0 + it.icount - 1;  // chunk meets the diagonal
 
+            // FP64 carry: wave sums of S = sum s, Si = sum w_i s, Sw = sum w_j s,
+            // Siw = sum w_i w_j s for the lane's q-points.
+            T r0[M], ri[M], rw[M], riw[M];
+#pragma unroll
+            for (int t = 0; t < M; ++t) ri[t] = rw[t] = riw[t] = T(0);
+            if constexpr (ONEACC) {
+                T acc0[Q], unused[Q];
+#pragma unroll
+                for (int k = 0; k < Q; ++k) acc0[k] = T(0);
+#define PRIPPS_PB1(M, S, B) pair_block<T, Q, UNIFORM, false, M, S, B>(atoms, j, jend, ai, iidx, lane_take, qv, ql, q0p, q0lo, hp, hlo, small_h, acc0, unused)
+#define PRIPPS_PASS(B)                                                                   \
+                if constexpr (UNIFORM && sizeof(T) == 4 && VARIANT == 4) {              \
+                    if (mask) { if (small_h) PRIPPS_PB1(true, 1, B); else PRIPPS_PB1(true, 0, B); } \
+                    else { if (small_h) PRIPPS_PB1(false, 1, B); else PRIPPS_PB1(false, 0, B); }    \
+                } else {                                                                 \
+                    if (mask) PRIPPS_PB1(true, -1, B);                                   \
+                    else PRIPPS_PB1(false, -1, B);                                       \
+                }
+                PRIPPS_PASS(false)
+                reduce_scatter<T, Q, false>(acc0, T(1), r0);
+                if (wet_i) reduce_scatter<T, Q, true>(acc0, wi, ri);
+                if (wet) {
+#pragma unroll
+                    for (int k = 0; k < Q; ++k) acc0[k] = T(0);
+                    PRIPPS_PASS(true)
+                    reduce_scatter<T, Q, false>(acc0, T(1), rw);
+                    if (wet_i) reduce_scatter<T, Q, true>(acc0, wi, riw);
+                }
+#undef PRIPPS_PASS
+#undef PRIPPS_PB1
+            } else {
             T acc0[Q], accw[Q];
 #pragma unroll
             for (int k = 0; k < Q; ++k) acc0[k] = accw[k] = T(0);
@@ -573,16 +640,12 @@ __global__ void __launch_bounds__(256) debye_tiled_kernel(TiledArgs args) {
 #undef PRIPPS_PB
             }
 
-            // FP64 carry: wave sums of S = sum s, Si = sum w_i s, Sw = sum w_j s,
-            // Siw = sum w_i w_j s for the lane's q-points.
-            T r0[M], ri[M], rw[M], riw[M];
             reduce_scatter<T, Q, false>(acc0, T(1), r0);
-#pragma unroll
-            for (int t = 0; t < M; ++t) ri[t] = rw[t] = riw[t] = T(0);
             if (wet_i) reduce_scatter<T, Q, true>(acc0, wi, ri);
             if (wet) {
                 reduce_scatter<T, Q, false>(accw, T(1), rw);
                 if (wet_i) reduce_scatter<T, Q, true>(accw, wi, riw);
+            }
             }
 #pragma unroll
             for (int t = 0; t < M; ++t) {
@@ -1009,9 +1072,16 @@ __global__ void combine_kernel(const double *__restrict__ prof, int nq, double v
 // ---------------------------------------------------------------------------
 // launch wrappers (host side of this translation unit)
 // ---------------------------------------------------------------------------
+// The one-accumulator kernel where it saves a wave's worth of registers: FP32
+// on uniform grids, variants 3 and 4 (FP64 and the non-uniform FP32 kernel
+// stay at two waves per SIMD either way, so they keep two arrays).
+template <typename T, int Q, bool U, int V>
+constexpr bool kOneAcc = sizeof(T) == 4 && U && V != 1 && V != 2;
+
 template <typename T, int Q, bool U, int V = 3>
 static hipError_t launch_tiled_t(const TiledArgs &a, int blocks, hipStream_t st) {
-    hipLaunchKernelGGL((debye_tiled_kernel<T, Q, U, V>), dim3(blocks), dim3(256), 0, st, a);
+    constexpr bool ONE = kOneAcc<T, Q, U, V>;
+    hipLaunchKernelGGL((debye_tiled_kernel<T, Q, U, V, ONE>), dim3(blocks), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
@@ -1039,14 +1109,18 @@ hipError_t launch_tiled(bool fp64, int Q, bool uniform, const TiledArgs &a, int 
     return hipErrorInvalidValue;
 }
 
-template <typename T, int Q, bool U>
+template <typename T, int Q, bool U, int V = 3>
 static hipError_t occupancy_t(int *blocks_per_cu) {
+    constexpr bool ONE = kOneAcc<T, Q, U, V>;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        blocks_per_cu, reinterpret_cast<const void *>(&debye_tiled_kernel<T, Q, U, 3>), 256, 0);
+        blocks_per_cu, reinterpret_cast<const void *>(&debye_tiled_kernel<T, Q, U, V, ONE>), 256, 0);
 }
 
-hipError_t tiled_occupancy(bool fp64, int Q, bool uniform, int *blocks_per_cu) {
+hipError_t tiled_occupancy(bool fp64, int Q, bool uniform, int variant, int *blocks_per_cu) {
     if (!fp64) {
+        if (uniform && Q == 64 && (variant == 1 || variant == 2))
+            return variant == 1 ? occupancy_t<float, 64, true, 1>(blocks_per_cu)
+                                : occupancy_t<float, 64, true, 2>(blocks_per_cu);
         if (Q == 32) return uniform ? occupancy_t<float, 32, true>(blocks_per_cu) : occupancy_t<float, 32, false>(blocks_per_cu);
         if (Q == 64) return uniform ? occupancy_t<float, 64, true>(blocks_per_cu) : occupancy_t<float, 64, false>(blocks_per_cu);
         if (Q == 128 && uniform)

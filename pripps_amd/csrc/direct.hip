@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <numeric>
 #include <string>
@@ -308,7 +309,53 @@ __global__ void __launch_bounds__(kGroupThreads) direct_group_kernel(GroupArgs a
     if (tid < 8) a.out[tid * a.G + g] = red[tid][0] / double(e - b);
 }
 
+// Trajectory frames: acc[v] += w (|A - vs E + cd H|^2, self) for one frame.
+__global__ void __launch_bounds__(256) direct_frames_accumulate_kernel(const double *amp, int V,
+                                                                       double vs, double cd,
+                                                                       double w, double *acc) {
+#pragma clang fp contract(off)
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const double *r = amp + (int64_t)v * kDirVals;
+    const double re = (r[0] - vs * r[2]) + cd * r[4];
+    const double im = (r[1] - vs * r[3]) + cd * r[5];
+    acc[2 * (int64_t)v] += w * (re * re + im * im);
+    acc[2 * (int64_t)v + 1] += w * r[6];
+}
+
+// out[t][g] = mean over group g of acc[v][t] / wsum (t = 0 total, 1 self);
+// the same fixed-order block reduction as direct_group_kernel.
+__global__ void __launch_bounds__(kGroupThreads) direct_frames_group_kernel(
+    const double *acc, const int32_t *group_ptr, const int32_t *group_vec, int G, double wsum,
+    double *out) {
+    __shared__ double red[2][kGroupThreads];
+    const int g = blockIdx.x;
+    const int tid = threadIdx.x;
+    double t0 = 0, t1 = 0;
+    const int b = group_ptr[g], e = group_ptr[g + 1];
+    for (int p = b + tid; p < e; p += kGroupThreads) {
+        t0 += acc[2 * (int64_t)group_vec[p]];
+        t1 += acc[2 * (int64_t)group_vec[p] + 1];
+    }
+    red[0][tid] = t0;
+    red[1][tid] = t1;
+    __syncthreads();
+    for (int half = kGroupThreads / 2; half > 0; half /= 2) {
+        if (tid < half) {
+            red[0][tid] += red[0][tid + half];
+            red[1][tid] += red[1][tid + half];
+        }
+        __syncthreads();
+    }
+    if (tid < 2) out[tid * G + g] = red[tid][0] / double(e - b) / wsum;
+}
+
 __global__ void direct_fill_int_kernel(int *p, int v) { *p = v; }
+
+__global__ void direct_fill_double_kernel(double *p, int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
 
 // ---------------------------------------------------------------------------
 // host
@@ -354,6 +401,10 @@ struct pripps_direct_model {
     double *d_amp = nullptr;
     double *d_out = nullptr;
     pripps_build_stats stats{};
+    // re-staging of positions (trajectory frames)
+    std::vector<int32_t> order;  // sorted atom -> input atom
+    double L[3] = {0, 0, 0};
+    char *d_atoms = nullptr;
 };
 
 namespace {
@@ -586,6 +637,8 @@ extern "C" int pripps_direct_model_build(pripps_engine *engine, const pripps_deb
     m->n_atoms = N;
     m->has_self = self_mode;
     m->group_q.assign(grid->group_q, grid->group_q + G);
+    m->order = order;
+    for (int d = 0; d < 3; ++d) m->L[d] = grid->box[d];
 
     std::lock_guard<std::mutex> lock(*acc.mu);
     auto abort_build = [&](int rc) {
@@ -604,6 +657,7 @@ extern "C" int pripps_direct_model_build(pripps_engine *engine, const pripps_deb
     if (err != hipSuccess) return abort_build(hip_fail(err, "hipMemcpyAsync(direct inputs)"));
 
     args.atoms = d + o_atoms;
+    m->d_atoms = d + o_atoms;
     args.segs = reinterpret_cast<const DirectSegment *>(d + o_segs);
     args.nvec = d + o_nvec;
     args.ffa = reinterpret_cast<const double *>(d + o_ffa);
@@ -766,5 +820,125 @@ extern "C" int pripps_direct_model_free(pripps_direct_model *m) {
     }
     delete m;
     engine_unref(e);
+    return PRIPPS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// trajectory frames (reference trajectory_average, src/trajectory.rs:167-251)
+// ---------------------------------------------------------------------------
+extern "C" int pripps_direct_frames_average(pripps_engine *engine,
+                                            const pripps_debye_input *topology,
+                                            const pripps_direct_grid *grid, int32_t n_frames,
+                                            const double *coords, const double *w,
+                                            int32_t stride, int32_t precision, double vs,
+                                            double cd, double *out_total, double *out_self) {
+    if (!engine || !topology || !grid)
+        return fail(PRIPPS_ERR_USAGE, "engine, topology and grid must be non-NULL");
+    if (n_frames < 0) return fail(PRIPPS_ERR_VALIDATION, "n_frames must be >= 0");
+    if (stride < 1) return fail(PRIPPS_ERR_VALIDATION, "stride must be >= 1");
+    if (!std::isfinite(vs) || !std::isfinite(cd))
+        return fail(PRIPPS_ERR_VALIDATION, "volume_scale and contrast_density must be finite");
+    const int64_t n = topology->n_atoms;
+    if (n < 0) return fail(PRIPPS_ERR_VALIDATION, "negative size");
+    if (n_frames > 0 && n > 0 && !coords) return fail(PRIPPS_ERR_USAGE, "coords is NULL");
+    if (grid->n_groups > 0 && !out_total) return fail(PRIPPS_ERR_USAGE, "out_total is NULL");
+    if (out_self && grid->num_molecules <= 0)
+        return fail(PRIPPS_ERR_USAGE, "the self term needs num_molecules > 0");
+    // weights: the total of all must be usable (read_weights), and so must the
+    // total of the frames the stride keeps
+    if (w) {
+        double sum = 0;
+        for (int32_t f = 0; f < n_frames; ++f) sum += w[f];
+        if (!std::isnormal(std::fabs(sum)))
+            return fail(PRIPPS_ERR_VALIDATION, "sum of weights is zero or infinite");
+    }
+    std::vector<int32_t> used;
+    double wsum = 0;
+    for (int32_t f = 0; f < n_frames; f += stride) {
+        used.push_back(f);
+        wsum += w ? w[f] : 1.0;
+    }
+    if (!std::isnormal(std::fabs(wsum))) {
+        char buf[64];
+        std::snprintf(buf, sizeof buf, "%g", wsum);
+        return fail(PRIPPS_ERR_VALIDATION, "weights of the " + std::to_string(used.size()) +
+                                               " averaged frames sum to " + buf +
+                                               "; cannot normalize");
+    }
+    for (size_t k = 0; k < (size_t)n_frames * n * 3; ++k)
+        if (!std::isfinite(coords[k])) return fail(PRIPPS_ERR_VALIDATION, "coordinates must be finite");
+
+    // frame 0 of the used set builds the model (plan, tables, first amplitudes)
+    std::vector<double> fx(n), fy(n), fz(n);
+    auto soa = [&](int32_t f) {
+        const double *c = coords + (size_t)f * n * 3;
+        for (int64_t i = 0; i < n; ++i) {
+            fx[i] = c[3 * i];
+            fy[i] = c[3 * i + 1];
+            fz[i] = c[3 * i + 2];
+        }
+    };
+    soa(used[0]);
+    pripps_debye_input in0 = *topology;
+    in0.x = fx.data();
+    in0.y = fy.data();
+    in0.z = fz.data();
+    pripps_direct_model *m = nullptr;
+    int rc = pripps_direct_model_build(engine, &in0, grid, precision, &m);
+    if (rc) return rc;
+    struct Guard {
+        pripps_direct_model *m;
+        double *acc = nullptr;
+        ~Guard() {
+            if (acc) {
+                (void)hipSetDevice(m->acc.device);
+                (void)hipFree(acc);
+            }
+            pripps_direct_model_free(m);
+        }
+    } guard{m};
+    const int V = m->V, G = m->G;
+    std::vector<double> hout((size_t)2 * G, 0.0);
+    if (V > 0) {
+        std::lock_guard<std::mutex> lock(*m->acc.mu);
+        hipStream_t st = m->acc.stream;
+        DCHK(hipSetDevice(m->acc.device));
+        DCHK(hipMalloc(&guard.acc, (size_t)V * 2 * sizeof(double)));
+        hipLaunchKernelGGL(direct_fill_double_kernel, dim3((2 * V + 255) / 256), dim3(256), 0, st,
+                           guard.acc, (int64_t)2 * V);
+        DCHK(hipGetLastError());
+        const size_t esz = m->fp64 ? sizeof(double) : sizeof(float);
+        std::vector<char> host((size_t)std::max<int64_t>(n, 1) * 4 * esz);
+        for (size_t k = 0; k < used.size(); ++k) {
+            const int32_t f = used[k];
+            if (k > 0) {
+                soa(f);
+                pripps_debye_input inf = *topology;
+                inf.x = fx.data();
+                inf.y = fy.data();
+                inf.z = fz.data();
+                if (m->fp64) stage_atoms<double>(&inf, m->L, m->order, host.data());
+                else stage_atoms<float>(&inf, m->L, m->order, host.data());
+                DCHK(hipMemcpyAsync(m->d_atoms, host.data(), (size_t)n * 4 * esz,
+                                    hipMemcpyHostToDevice, st));
+                DCHK(enqueue(*m, st, nullptr, nullptr));
+            }
+            hipLaunchKernelGGL(direct_frames_accumulate_kernel, dim3((V + 255) / 256), dim3(256), 0,
+                               st, m->d_amp, V, vs, cd, w ? w[f] : 1.0, guard.acc);
+            DCHK(hipGetLastError());
+            // the host buffer is rewritten for the next frame
+            DCHK(hipStreamSynchronize(st));
+        }
+        if (G > 0) {
+            hipLaunchKernelGGL(direct_frames_group_kernel, dim3(G), dim3(kGroupThreads), 0, st,
+                               guard.acc, m->gargs.group_ptr, m->gargs.group_vec, G, wsum, m->d_out);
+            DCHK(hipGetLastError());
+            DCHK(hipMemcpyAsync(hout.data(), m->d_out, hout.size() * sizeof(double),
+                                hipMemcpyDeviceToHost, st));
+        }
+        DCHK(hipStreamSynchronize(st));
+    }
+    if (G > 0) std::copy(hout.begin(), hout.begin() + G, out_total);
+    if (out_self) std::copy(hout.begin() + G, hout.end(), out_self);
     return PRIPPS_OK;
 }

@@ -314,12 +314,16 @@ int prepare(const pripps_debye_input *in, Prepared &P) {
         cy /= (double)n;
         cz /= (double)n;
     }
-    // classes: (dense kind, wet), counting sort keeps the input order inside a class
+    // classes: every hydrated class (by dense kind) before every dry one.
+    // Pairs are visited as i < j, so a hydrated j atom then only meets the
+    // hydrated atoms before it: the pair terms that need the w_j sums drop
+    // from about f (the hydrated fraction of atoms) to f^2 / 2 of all pairs.
+    // The counting sort keeps the input order inside a class.
     const int C = 2 * K;
     std::vector<int64_t> count(C + 1, 0);
     auto cls_of = [&](int64_t i) {
         const double w = P.has_h ? in->hydration_weight[i] : 0.0;
-        return 2 * dense[in->kind[i]] + (w != 0.0 ? 1 : 0);
+        return (w != 0.0 ? 0 : K) + dense[in->kind[i]];
     };
     for (int64_t i = 0; i < n; ++i) count[cls_of(i) + 1]++;
     for (int c = 0; c < C; ++c) count[c + 1] += count[c];
@@ -335,20 +339,21 @@ int prepare(const pripps_debye_input *in, Prepared &P) {
         P.pos[4 * d + 1] = in->y[i] - cy;
         P.pos[4 * d + 2] = in->z[i] - cz;
         P.pos[4 * d + 3] = w;
-        P.kind_n[c / 2] += 1.0;
-        P.kind_w[c / 2] += w;
+        P.kind_n[c % K] += 1.0;
+        P.kind_w[c % K] += w;
         P.sum_w2 += w * w;
     }
     P.n_classes = C;
     for (int c = 0; c <= C; ++c) P.cls_start[c] = (int32_t)count[c];
     for (int c = 0; c < C; ++c) {
-        P.cls_kind[c] = (int8_t)(c / 2);
-        P.cls_wet[c] = (int8_t)(c % 2);
+        P.cls_kind[c] = (int8_t)(c % K);
+        P.cls_wet[c] = (int8_t)(c < K ? 1 : 0);
     }
-    // kind-pure i-tiles of up to 64 atoms
-    for (int k = 0; k < K; ++k) {
-        const int32_t b = P.cls_start[2 * k], e = P.cls_start[2 * k + 2];
-        for (int32_t i0 = b; i0 < e; i0 += 64) P.tiles.push_back(make_int4(i0, std::min(64, e - i0), k, 0));
+    // class-pure i-tiles of up to 64 atoms
+    for (int c = 0; c < C; ++c) {
+        const int32_t b = P.cls_start[c], e = P.cls_start[c + 1];
+        for (int32_t i0 = b; i0 < e; i0 += 64)
+            P.tiles.push_back(make_int4(i0, std::min(64, e - i0), c % K, 0));
     }
     // uniform grid detection (q_m = q0 + m dq to a few ulps)
     if (nq >= 1) {
@@ -710,7 +715,7 @@ static int make_launches(pripps_engine *e, const BuildPlan &B, char *dbase,
         a.partials = (double *)(dbase + g.o_partials);
         a.variant = g.variant;
         int bpc = 1;
-        HIPCHK(tiled_occupancy(B.fp64, g.Q, g.uniform, &bpc));
+        HIPCHK(tiled_occupancy(B.fp64, g.Q, g.uniform, g.variant, &bpc));
         bpc = std::max(1, bpc);
         const int64_t want = (r.n_items + 3) / 4;
         r.blocks = (int)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)e->n_cu * bpc));

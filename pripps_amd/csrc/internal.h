@@ -100,7 +100,7 @@ struct EngineAccess {
 // launch wrappers (debye_kernels.hip)
 hipError_t launch_tiled(bool fp64, int Q, bool uniform, const TiledArgs &a, int blocks,
                         hipStream_t st);
-hipError_t tiled_occupancy(bool fp64, int Q, bool uniform, int *blocks_per_cu);
+hipError_t tiled_occupancy(bool fp64, int Q, bool uniform, int variant, int *blocks_per_cu);
 hipError_t launch_finalize(const FinalizeArgs &a, int n_strips_total, hipStream_t st);
 hipError_t launch_plain(const PlainArgs &a, int nq, hipStream_t st);
 hipError_t launch_fill_int(int *p, int n, int v, hipStream_t st);

@@ -396,15 +396,43 @@ class NativeDirectModel:
             pass
 
 
-def build_native(engine, inp: N.PreparedInput, box, indices, group, group_q,
-                 num_molecules: int = 0, precision: str = "fp32") -> NativeDirectModel:
-    prec = N.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+def _grid_struct(box, indices, group, group_q, num_molecules):
     idx = np.ascontiguousarray(np.asarray(indices, dtype=np.int32).reshape(-1, 3))
     grp = np.ascontiguousarray(group, dtype=np.int32)
     gq = np.ascontiguousarray(group_q, dtype=np.float64)
     grid = N.DirectGrid((ctypes.c_double * 3)(*[float(b) for b in box]), len(idx),
                         idx.ctypes.data_as(N._I32), int(num_molecules), len(gq),
                         grp.ctypes.data_as(N._I32), gq.ctypes.data_as(N._D))
+    return grid, (idx, grp, gq)
+
+
+def frames_average(engine, inp: N.PreparedInput, box, indices, group, group_q, frames,
+                   weights=None, stride: int = 1, num_molecules: int = 0,
+                   precision: str = "fp32", volume_scale: float = 1.0,
+                   contrast_density: float = 0.0):
+    """(total [G], self [G] or None): weighted average over frames
+    (``pripps_direct_frames_average``)."""
+    prec = N.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    grid, keep = _grid_struct(box, indices, group, group_q, num_molecules)
+    f = np.ascontiguousarray(frames, dtype=np.float64)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
+    if w is not None and len(w) != len(f):
+        from .errors import ValidationError
+        raise ValidationError(f"{len(w)} weights for {len(f)} frames")
+    G = len(keep[2])
+    total = np.zeros(G)
+    self_term = np.zeros(G) if num_molecules else None
+    N.check(N.load().pripps_direct_frames_average(
+        engine.handle, ctypes.byref(inp.c), ctypes.byref(grid), len(f), N.dptr(f), N.dptr(w),
+        int(stride), prec, float(volume_scale), float(contrast_density), N.dptr(total),
+        N.dptr(self_term)))
+    return total, self_term
+
+
+def build_native(engine, inp: N.PreparedInput, box, indices, group, group_q,
+                 num_molecules: int = 0, precision: str = "fp32") -> NativeDirectModel:
+    prec = N.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    grid, keep = _grid_struct(box, indices, group, group_q, num_molecules)
     h = ctypes.c_void_p()
     N.check(N.load().pripps_direct_model_build(engine.handle, ctypes.byref(inp.c),
                                                ctypes.byref(grid), prec, ctypes.byref(h)))

@@ -240,3 +240,60 @@ def test_uneven_molecule_chunks(engine, oracle, tmp_path, precision):
         es = _rel(m.self_term(), _grouped(rows[:, 6], group, len(gq)))
         print(f"direct chunks n={n} N={nmol} {precision}: total {et:.3e} self {es:.3e}")
         assert max(et, es) <= TOL[precision]
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_direct_trajectory_average(engine, oracle, tmp_path, precision):
+    """trajectory_average (src/trajectory.rs:167-251) with the Direct scheme:
+    frames 0, s, 2s, ... weighted by the original frame index, q grid of the
+    fixed box; against the weighted mean of per-frame oracle curves and of
+    per-frame GPU builds."""
+    import pripps_amd
+    from pripps_amd.direct import QGrid
+    yaml = tmp_path / "xy.yaml"
+    yaml.write_text("X: { formfactor: !Constant 1.0 }\nY: { formfactor: !Constant 0.5 }\n")
+    rng = np.random.default_rng(31)
+    mol = rng.uniform(-3, 3, size=(5, 3))
+    kinds = ["X", "Y", "X", "X", "Y"]
+    n_mol = 6
+    xyz = tmp_path / "box.xyz"
+    base = np.vstack([mol + rng.uniform(0, 50, 3) for _ in range(n_mol)])
+    xyz.write_text(f"{len(base)}\nbox\n" + "".join(
+        f"{k} {p[0]} {p[1]} {p[2]}\n" for k, p in zip(kinds * n_mol, base)))
+    frames = np.stack([base + rng.normal(0, 0.7, base.shape) for _ in range(7)])
+    weights = rng.uniform(0.2, 2.0, size=7)
+    p = pripps_amd.Pripps(str(yaml), engine=engine)
+    box = (60.0, 60.0, 60.0)
+    res = p.direct_trajectory_average(str(xyz), frames, box_sides=60.0, pmax=5,
+                                      num_molecules=n_mol, weights=weights, stride=2,
+                                      precision=precision)
+    idx, qn, group, gq = QGrid.miller(5).groups(box)
+    ids = np.array([0 if k == "X" else 1 for k in kinds * n_mol], dtype=np.int32)
+    ffa = np.array([np.full(len(idx), 1.0), np.full(len(idx), 0.5)])
+    tot = np.zeros(len(gq))
+    slf = np.zeros(len(gq))
+    gpu_tot = np.zeros(len(gq))
+    wsum = 0.0
+    for f in range(0, 7, 2):
+        rows = oracle.direct_rows(frames[f], ids, box, idx, ffa, num_molecules=n_mol)
+        tot += weights[f] * _grouped(rows[:, 0] ** 2 + rows[:, 1] ** 2, group, len(gq))
+        slf += weights[f] * _grouped(rows[:, 6], group, len(gq))
+        one = p.direct_model_from_arrays(frames[f], ids, pmax=5, box_sides=60.0,
+                                         precision=precision)
+        gpu_tot += weights[f] * one.intensity(1.0, 0.0).total
+        wsum += weights[f]
+    et = _rel(res.total, tot / wsum)
+    es = _rel(res.p * n_mol, slf / wsum)
+    eg = _rel(res.total, gpu_tot / wsum)
+    print(f"direct frames {precision}: total {et:.3e} self {es:.3e} vs per-frame builds {eg:.3e}")
+    assert max(et, es) <= TOL[precision]
+    assert eg <= (1e-13 if precision == "fp64" else 1e-6)
+    assert np.array_equal(res.q, gq)
+    # validation: the stride keeps frames whose weights sum to zero
+    with pytest.raises(pripps_amd.ValidationError, match="cannot normalize"):
+        p.direct_trajectory_average(str(xyz), frames[:3], box_sides=60.0, pmax=2,
+                                    weights=np.array([1.0, 5.0, -1.0]), stride=2)
+    with pytest.raises(pripps_amd.ValidationError):
+        p.direct_trajectory_average(str(xyz), frames[:, :-1], box_sides=60.0, pmax=2)
+    with pytest.raises(ValueError):
+        p.direct_trajectory_average(str(xyz), frames, box_sides=60.0, pmax=2, stride=0)
