@@ -214,7 +214,8 @@ def main():
                    "block": st.block, "span": st.span, "work_items": st.work_items},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": "debye_tiled_kernel<float,%d,true>" % st.q_per_lane,
+                     "kernel": ("debye_dual_kernel" if st.q_per_lane == 128
+                                else "debye_tiled_kernel<float,%d,true,3,true>" % st.q_per_lane),
                      "kernel_ms": pair_ms, "flop_per_term": FLOP_PER_TERM_SOLVENT,
                      "executed": executed},
         "cpu_baseline": cpu,

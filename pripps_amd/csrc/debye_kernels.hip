@@ -682,13 +682,16 @@ __global__ void __launch_bounds__(256) debye_tiled_kernel(TiledArgs args) {
 
 // ---------------------------------------------------------------------------
 // FP32, uniform grid, 128 q points per item as two 64-point halves.
-//   dry j class: per pair the geometry and the half-step terms (r, 1/r,
-//     sin h, cos h - 1, lambda) are computed once and seed both halves, each
-//     a 64-step recurrence into its own accumulators;
+//   dry j class: per pair one seed starts a 128-step recurrence whose first
+//     and second 64 steps go to the two halves' accumulators;
 //   hydrated j class: the halves run one after the other (acc0 = a, accw = b),
 //     with the 64-point pair block.
-// Every half-strip sees the same seeds, recurrence and carry as the 64-point
-// kernel, so the halves' partials equal that kernel's item partials.
+// The first half of a dry pair and both halves of a hydrated pair see the
+// same seeds, recurrence and carry as the 64-point kernel; the second half of
+// a dry pair continues the recurrence for 64 more steps instead of starting
+// from a fresh seed.  The FP64 profile sums live in the item's own partial
+// slot (L2-resident, written only by this wave): in registers they would push
+// the kernel past 256 VGPRs, to one wave per SIMD.
 // ---------------------------------------------------------------------------
 template <bool MASK>
 __device__ __forceinline__ void dual_block_dry(const Atom<float> *__restrict__ atoms, int jb, int je,
@@ -705,25 +708,21 @@ __device__ __forceinline__ void dual_block_dry(const Atom<float> *__restrict__ a
         const float r2 = dx * dx + dy * dy + dz * dz;
         const float take = MASK ? (j > iidx ? 1.0f : 0.0f) : lane_take;
         const PairGeom32 g = pair_geom_f32(r2, hp, hlo, take, small_h);
-        {
-            const Seed32 z = seed_from_geom(g, q0p[0], q0lo[0]);
-            float sv = z.s, d = z.d;
+        // one seed per pair: the second half continues the first half's
+        // recurrence (its state after 64 steps is the second half's seed)
+        const Seed32 z = seed_from_geom(g, q0p[0], q0lo[0]);
+        float sv = z.s, d = z.d;
 #pragma unroll
-            for (int k = 0; k < 64; ++k) {
-                acc_a[k] += sv;
-                sv += d;
-                d -= z.lam * sv;
-            }
+        for (int k = 0; k < 64; ++k) {
+            acc_a[k] += sv;
+            sv += d;
+            d -= z.lam * sv;
         }
-        {
-            const Seed32 z = seed_from_geom(g, q0p[1], q0lo[1]);
-            float sv = z.s, d = z.d;
 #pragma unroll
-            for (int k = 0; k < 64; ++k) {
-                acc_b[k] += sv;
-                sv += d;
-                d -= z.lam * sv;
-            }
+        for (int k = 0; k < 64; ++k) {
+            acc_b[k] += sv;
+            sv += d;
+            d -= z.lam * sv;
         }
     }
 }

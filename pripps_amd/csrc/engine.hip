@@ -406,9 +406,11 @@ int valid_q(int precision, bool uniform, int Q) {
     return (Q == 32 || Q == 64 || (Q == 128 && uniform)) ? Q : 0;
 }
 
+// FP32 on a uniform grid: 128-point strips (the dual kernel: one recurrence
+// seed per 128 q-points of a dry pair) unless the grid is short.
 int default_q(int precision, bool uniform, int nq) {
     if (precision == PRIPPS_FP64 || !uniform) return 32;
-    return nq <= 32 ? 32 : 64;
+    return nq <= 32 ? 32 : nq <= 64 ? 64 : 128;
 }
 
 // j atoms per work item: about 16 items per wave slot of the GPU when the
