@@ -33,15 +33,15 @@ UNIT = "pair·q terms/s"
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
 FLOP_PER_TERM_SOLVENT = 14.0
-# Counter figures of the pair kernel on the default solvent10k plan (Q = 64,
-# span = 256, 25,344 items), from profiles/debye_tiled_f32_solvent10k.md:
+# Counter figures of the pair kernel on the default solvent10k plan (Q = 128,
+# span = 256, 12,928 items), from profiles/debye_dual_f32_solvent10k.md:
 #   bytes: HBM bytes per launch, rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
 #          (scale factors from a plain-copy calibration);
 #   executed_flop_per_term: rocprofv3 TOTAL_32_OPS ((2 FMA + ADD + MUL + TRANS +
 #          INT32) x 64 per dispatch) / pair-q terms.
-COUNTERS_SOLVENT10K = {"q_per_lane": 64, "span": 256, "work_items": 25344,
-                       "bytes": (3051.3 * 2.0 + 76888.0 * 1.0) * 1024,
-                       "executed_flop_per_term": 1.552e11 / 2.49975e10}
+COUNTERS_SOLVENT10K = {"q_per_lane": 128, "span": 256, "work_items": 12928,
+                       "bytes": (41444.25 * 2.0 + 198713.5 * 1.0) * 1024,
+                       "executed_flop_per_term": 1.2833e11 / 2.49975e10}
 PAIRS = ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))
 
 

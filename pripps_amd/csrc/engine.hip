@@ -407,22 +407,26 @@ int valid_q(int precision, bool uniform, int Q) {
 }
 
 // FP32 on a uniform grid: 128-point strips (the dual kernel: one recurrence
-// seed per 128 q-points of a dry pair) unless the grid is short.
-int default_q(int precision, bool uniform, int nq) {
+// seed per 128 q-points of a dry pair) unless the grid is short or the
+// structure small (below ~2,000 atoms its items would not fill the GPU).
+int default_q(int precision, bool uniform, int nq, int n_atoms) {
     if (precision == PRIPPS_FP64 || !uniform) return 32;
-    return nq <= 32 ? 32 : nq <= 64 ? 64 : 128;
+    if (nq <= 32) return 32;
+    return (nq <= 64 || n_atoms < 2048) ? 64 : 128;
 }
 
 // j atoms per work item: about 16 items per wave slot of the GPU when the
-// structure runs alone, 256..8192, a multiple of 64.  It depends on the
-// structure only, so a structure is cut into the same items alone or in any
-// batch, and its profiles are bitwise the same either way.
+// structure runs alone, 64..8192, a multiple of 64 (256 and up from mid-size
+// structures on).  It depends on the structure only, so a structure is cut
+// into the same items alone or in any batch, and its profiles are bitwise
+// the same either way.
 int default_span(const Prepared &p, int Q, int n_cu) {
     const double strips = (double)((p.nq + Q - 1) / Q);
     const double work = (double)p.tiles.size() * 0.5 * p.n * strips;
     const double slots = std::max(1, n_cu) * 8.0;
     const int span = (int)(work / (16.0 * slots));
-    return std::max(256, std::min(8192, (span + 63) / 64 * 64));
+    const int floor = p.n >= 4096 ? 256 : 64;
+    return std::max(floor, std::min(8192, (span + 63) / 64 * 64));
 }
 
 // j atoms summed in FP32 per lane before the FP64 carry.  Longer blocks are
@@ -500,7 +504,7 @@ static StructPlan resolve_plan(pripps_engine *e, const pripps_tuning *tune, int 
     }
     int Q = expl ? valid_q(precision, p.uniform, tune->q_per_lane)
                  : have ? valid_q(precision, p.uniform, te.q_per_lane) : 0;
-    if (!Q) Q = default_q(precision, p.uniform, p.nq);
+    if (!Q) Q = default_q(precision, p.uniform, p.nq, p.n);
     sp.Q = Q;
     const int base = default_span(p, Q, e->n_cu);
     if (expl && tune->span > 0) sp.span = std::max(64, tune->span);
@@ -1170,7 +1174,7 @@ extern "C" int pripps_engine_autotune(pripps_engine *e, const pripps_debye_input
         return PRIPPS_OK;
     };
 
-    Cand cur{default_q(precision, p.uniform, p.nq), 0, kDefaultBlock, 3};
+    Cand cur{default_q(precision, p.uniform, p.nq, p.n), 0, kDefaultBlock, 3};
     double cur_ms = 0;
     rc = eval(cur, cur_ms);
     if (rc) return rc;
