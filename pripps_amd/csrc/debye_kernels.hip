@@ -1,7 +1,9 @@
 // HIP kernels of the exact Debye engine (gfx950).
 //
 // Work decomposition
-//   Atoms are sorted by (kind, hydrated) into class segments.  A work item is
+//   Atoms are sorted into class segments, hydrated classes first, then by
+//   kind (a hydrated j atom therefore only meets the hydrated i atoms before
+//   it, so few pairs need the w_j sums).  A work item is
 //   one wave: a 64-atom i-tile of a single kind (one atom per lane) against a
 //   span of j atoms, for one strip of Q q-points held by every lane.  Only
 //   pairs with j > i are taken, so each unordered pair of a structure is
@@ -15,9 +17,11 @@
 //   sin(a + kd) - sin(a + (k-1)d) written in Reinsch's difference form: the
 //   constant lambda keeps its full relative accuracy for small d, where the
 //   plain 2 cos(d) coefficient would lose it, so the FP32 phase does not drift.
-//   Every strip is seeded afresh (polynomial sincos, residual-corrected r and
-//   1/r; see seed_f32), so the recurrence is refreshed every Q steps.  On
-//   non-uniform grids each term takes a direct hardware sin instead.
+//   Seeds use a polynomial sincos and residual-corrected r and 1/r (see
+//   seed_f32).  The tiled kernel seeds every strip afresh; the dual kernel
+//   (FP32, 128 q-points per item) seeds a dry pair once and lets the
+//   recurrence run on through both 64-point halves.  On non-uniform grids
+//   each term takes a direct hardware sin instead.
 //
 // Accumulation
 //   Lanes sum at most `block` pairs in FP32 (acc0/accw), then a cross-lane
