@@ -72,6 +72,8 @@ def main():
     ap.add_argument("--span", type=int, default=0)
     ap.add_argument("--variant", type=int, default=0)
     ap.add_argument("--cpu-q", type=int, default=8, help="q points of the CPU baseline sample")
+    ap.add_argument("--cpu-q1", type=int, default=1,
+                    help="q points of the one-core CPU baseline sample (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--workload", default="solvent10k",
@@ -79,8 +81,9 @@ def main():
     ap.add_argument("--cpu-vectors", type=int, default=256,
                     help="direct: q-vectors of the oracle / CPU baseline sample")
     ap.add_argument("--count", type=int, default=512, help="batch512: structures")
-    ap.add_argument("--plain-sample", type=int, default=32,
-                    help="batch512: structures timed on the sequential plain FP64 path")
+    ap.add_argument("--plain-sample", type=int, default=512,
+                    help="batch512: structures timed on the sequential plain FP64 path "
+                         "(all of them by default)")
     ap.add_argument("--rates", default="10,50,100,200,500,1000,2000")
     ap.add_argument("--requests", type=int, default=100, help="served: requests per rate")
     ap.add_argument("--total-requests", type=int, default=0,
@@ -132,6 +135,11 @@ def main():
         fp64_prof = m64t.profiles()
         pm, _ = m64t.recompute(2)
         extra["fp64_tiled_pair_kernel_ms"] = pm
+        tf64 = FLOP_PER_TERM_SOLVENT * terms / (pm * 1e-3) / 1e12
+        extra["fp64_roofline"] = {"bound": "valu", "achieved": tf64, "peak": FP64_PEAK_TFLOPS,
+                                  "unit": "TFLOP/s", "frac": tf64 / FP64_PEAK_TFLOPS,
+                                  "q_per_lane": m64t.stats().q_per_lane,
+                                  "flop_per_term": FLOP_PER_TERM_SOLVENT}
         extra["fp64_tiled_vs_plain_maxerr"] = envelope_error(fp64_prof, plain_prof)
         log(f"fp64 tiled pair kernel {pm:.2f} ms; tiled vs plain {extra['fp64_tiled_vs_plain_maxerr']:.2e}")
         del m64, m64t
@@ -191,6 +199,20 @@ def main():
         err_oracle = envelope_error(prof32[:, sel], ref)
         log(f"cpu baseline: {cpu['value']:.3e} terms/s on {threads} threads ({tcpu:.1f} s); "
             f"fp32 vs oracle {err_oracle:.2e}")
+        if args.cpu_q1 > 0:
+            # the same loop on one core, over fewer q
+            sel1 = sel[len(sel) // 2: len(sel) // 2 + args.cpu_q1]
+            tc = time.perf_counter()
+            O.build_model(wl.pos, inp.kind, inp.q[sel1], inp.atom_ff[:, sel1],
+                          inp.excluded_ff[:, sel1], inp.water_ff[sel1], inp.weights,
+                          mode=O.REFERENCE_ORDER, nthreads=1)
+            t1c = time.perf_counter() - tc
+            extra["cpu_baseline_1core"] = {
+                "value": wl.n_atoms * (wl.n_atoms - 1) / 2 * len(sel1) / t1c, "unit": UNIT,
+                "cores": 1, "kind": "port",
+                "sample": f"same loop, {len(sel1)} q, {t1c:.1f} s"}
+            log(f"cpu baseline, 1 core: {extra['cpu_baseline_1core']['value']:.3e} terms/s "
+                f"({t1c:.1f} s)")
 
     plan = {"q_per_lane": st.q_per_lane, "span": st.span, "work_items": st.work_items}
     same_plan = all(COUNTERS_SOLVENT10K[k] == v for k, v in plan.items())
@@ -390,17 +412,24 @@ def bench_batch512(args):
     errs = [envelope_error(a.profiles(), b.profiles()) for a, b in zip(models, m64)]
     out["fp32_vs_fp64_max_err"] = float(max(errs))
     del m64
-    # sequential plain FP64 launches on an evenly spaced sample, extrapolated by terms
+    # sequential plain FP64 launches, one per structure (all of them, or an
+    # evenly spaced sample extrapolated by terms)
     k = min(args.plain_sample, len(wls))
     sel = np.unique(np.linspace(0, len(wls) - 1, k).round().astype(int))
     plain_ms, plain_terms = 0.0, 0.0
-    for s in sel:
+    tw = time.perf_counter()
+    for n_done, s in enumerate(sel):
         m = eng.build(inps[s], "fp64_plain")
         plain_ms += m.stats().kernel_ms
         plain_terms += wls[s].pair_q_terms
+        del m
+        if (n_done + 1) % 128 == 0:
+            log(f"plain sequential: {n_done + 1} of {len(sel)} structures")
+    plain_wall = (time.perf_counter() - tw) * 1e3
     plain_rate = plain_terms / (plain_ms * 1e-3)
     out["fp64_plain_sequential"] = {"sample": int(len(sel)), "sample_terms": plain_terms,
-                                    "sample_kernel_ms": plain_ms, "terms_per_s": plain_rate,
+                                    "sample_kernel_ms": plain_ms, "sample_wall_ms": plain_wall,
+                                    "terms_per_s": plain_rate,
                                     "extrapolated_ms": terms / plain_rate * 1e3}
     out["speedup_vs_sequential_plain"] = out["fp32_batch"]["terms_per_s"] / plain_rate
     log(f"plain sequential sample: {out['fp64_plain_sequential']}; speedup {out['speedup_vs_sequential_plain']:.1f}x")

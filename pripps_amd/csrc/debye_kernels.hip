@@ -497,7 +497,7 @@ __device__ __forceinline__ void pair_block(const Atom<T> *__restrict__ atoms, in
 // VGPRs and runs three waves per SIMD instead of two, which lets the VALU
 // dual-issue far more often.
 template <typename T, int Q, bool UNIFORM, int VARIANT, bool ONEACC = false>
-__global__ void __launch_bounds__(256) debye_tiled_kernel(TiledArgs args) {
+__device__ __forceinline__ void tiled_body(const TiledArgs &args) {
     constexpr bool INTERLEAVE = VARIANT == 2;
     constexpr int M = Split<Q>::M;
     const int lane = __lane_id();
@@ -682,6 +682,21 @@ __global__ void __launch_bounds__(256) debye_tiled_kernel(TiledArgs args) {
             }
         }
     }
+}
+
+template <typename T, int Q, bool UNIFORM, int VARIANT, bool ONEACC = false>
+__global__ void __launch_bounds__(256) debye_tiled_kernel(TiledArgs args) {
+    tiled_body<T, Q, UNIFORM, VARIANT, ONEACC>(args);
+}
+
+// FP64, uniform grid, 64 q-points per item, one accumulator array: 64 FP64
+// accumulators (128 VGPRs).  Left to itself the compiler uses 258 registers,
+// one wave per SIMD; capped at two waves it keeps the loops spill-free (one
+// spilled value, reloaded once per item).  One seed per pair serves 64
+// q-points instead of 32, which is where the FP64 path spends its time.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+debye_tiled_f64_q64_kernel(TiledArgs args) {
+    tiled_body<double, 64, true, 3, true>(args);
 }
 
 // ---------------------------------------------------------------------------
@@ -1076,8 +1091,9 @@ __global__ void combine_kernel(const double *__restrict__ prof, int nq, double v
 // launch wrappers (host side of this translation unit)
 // ---------------------------------------------------------------------------
 // The one-accumulator kernel where it saves a wave's worth of registers: FP32
-// on uniform grids, variants 3 and 4 (FP64 and the non-uniform FP32 kernel
-// stay at two waves per SIMD either way, so they keep two arrays).
+// on uniform grids, variants 3 and 4 (the non-uniform FP32 kernel and FP64 at
+// Q = 32 stay at two waves per SIMD either way, so they keep two arrays).
+// FP64 at Q = 64 has its own entry point, debye_tiled_f64_q64_kernel.
 template <typename T, int Q, bool U, int V>
 constexpr bool kOneAcc = sizeof(T) == 4 && U && V != 1 && V != 2;
 
@@ -1107,7 +1123,10 @@ hipError_t launch_tiled(bool fp64, int Q, bool uniform, const TiledArgs &a, int 
     } else {
         if (Q == 32) return uniform ? launch_tiled_t<double, 32, true>(a, blocks, st)
                                     : launch_tiled_t<double, 32, false>(a, blocks, st);
-        if (Q == 64 && uniform) return launch_tiled_t<double, 64, true>(a, blocks, st);
+        if (Q == 64 && uniform) {
+            hipLaunchKernelGGL(debye_tiled_f64_q64_kernel, dim3(blocks), dim3(256), 0, st, a);
+            return hipGetLastError();
+        }
     }
     return hipErrorInvalidValue;
 }
@@ -1131,7 +1150,9 @@ hipError_t tiled_occupancy(bool fp64, int Q, bool uniform, int variant, int *blo
                 blocks_per_cu, reinterpret_cast<const void *>(&debye_dual_kernel), 256, 0);
     } else {
         if (Q == 32) return uniform ? occupancy_t<double, 32, true>(blocks_per_cu) : occupancy_t<double, 32, false>(blocks_per_cu);
-        if (Q == 64 && uniform) return occupancy_t<double, 64, true>(blocks_per_cu);
+        if (Q == 64 && uniform)
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                blocks_per_cu, reinterpret_cast<const void *>(&debye_tiled_f64_q64_kernel), 256, 0);
     }
     return hipErrorInvalidValue;
 }

@@ -406,12 +406,14 @@ int valid_q(int precision, bool uniform, int Q) {
     return (Q == 32 || Q == 64 || (Q == 128 && uniform)) ? Q : 0;
 }
 
-// FP32 on a uniform grid: 128-point strips (the dual kernel: one recurrence
-// seed per 128 q-points of a dry pair) unless the grid is short or the
-// structure small (below ~2,000 atoms its items would not fill the GPU).
+// Uniform grids: the longest strip whose accumulators fit, so that one
+// recurrence seed serves more q-points -- FP32 128 (the dual kernel), FP64 64
+// -- unless the grid is short or the structure small (below ~2,000 atoms
+// longer strips leave too few items to fill the GPU).  Non-uniform grids
+// take a direct sine per term and keep 32.
 int default_q(int precision, bool uniform, int nq, int n_atoms) {
-    if (precision == PRIPPS_FP64 || !uniform) return 32;
-    if (nq <= 32) return 32;
+    if (!uniform || nq <= 32) return 32;
+    if (precision == PRIPPS_FP64) return n_atoms < 2048 ? 32 : 64;
     return (nq <= 64 || n_atoms < 2048) ? 64 : 128;
 }
 

@@ -15,7 +15,9 @@
 //   fma literal / const literal and inline-constant operands
 //   sin, mix            v_sin_f32 chains, and 2 transcendentals + 6 FMAs: the
 //                       Direct kernel's mix
-//   occupancy sweep     add (VGPR) and the recurrence at 1..4 waves per SIMD,
+//   recurrence x1 / x2  one or two dependent chains per lane (the dual
+//                       kernel's dry loop runs one pair, i.e. one chain)
+//   occupancy sweep     add (VGPR) and the recurrences at 1..4 waves per SIMD,
 //                       limited with dynamic LDS
 #include <hip/hip_runtime.h>
 
@@ -97,6 +99,32 @@ PROBE_KERNEL(k_rec) {
         }
     float t = 0;
     for (int c = 0; c < 4; ++c) t += acc[c] + s[c];
+    out[blockIdx.x * 256 + threadIdx.x] = t + b;
+}
+
+// the recurrence with NC independent chains per lane (one chain = one pair of
+// the dual kernel's dry loop)
+template <int NC>
+__global__ void __launch_bounds__(256) k_rec_n(float *out, float a, float b) {
+    float s[NC], d[NC], l[NC], acc[4];
+    for (int c = 0; c < NC; ++c) {
+        s[c] = threadIdx.x * 1e-3f + c;
+        d[c] = 1e-3f * c;
+        l[c] = a * 1e-4f * (c + 1);
+    }
+    for (int u = 0; u < 4; ++u) acc[u] = 0;
+    for (int i = 0; i < IT / NC; ++i)
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                acc[u] += s[c];
+                s[c] += d[c];
+                d[c] = __builtin_fmaf(-l[c], s[c], d[c]);
+            }
+    float t = 0;
+    for (int c = 0; c < NC; ++c) t += s[c];
+    for (int u = 0; u < 4; ++u) t += acc[u];
     out[blockIdx.x * 256 + threadIdx.x] = t + b;
 }
 
@@ -199,6 +227,8 @@ int main() {
         {"recurrence x4", k_rec, 3.0 * 4 * IT},  {"fma literal", k_fma_lit, chain},
         {"fma inline const", k_fma_ic, chain},   {"sin", k_sin, 2.0 * chain},
         {"mix 2 trans + 6 fma", k_mix, 8.0 * CH * (IT / 4)},
+        {"recurrence x1", k_rec_n<1>, 3.0 * 4 * IT},
+        {"recurrence x2", k_rec_n<2>, 3.0 * 4 * IT},
     };
     int rc = 0;
     for (const auto &p : probes) rc |= run(p, out, 0, "8 waves/SIMD");
@@ -207,7 +237,7 @@ int main() {
         {"1 wave/SIMD", 96 * 1024}, {"2 waves/SIMD", 64 * 1024}, {"3 waves/SIMD", 48 * 1024},
         {"4 waves/SIMD", 36 * 1024}};
     for (const auto &o : occ)
-        for (const Probe &p : {probes[2], probes[4]}) {
+        for (const Probe &p : {probes[2], probes[4], probes[9], probes[10]}) {
             if (hipFuncSetAttribute(reinterpret_cast<const void *>(p.f),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)o.lds) != hipSuccess)
                 return 1;
