@@ -1,4 +1,5 @@
-"""Fit the polynomial coefficients of ``sincos_rev`` in debye_kernels.hip.
+"""Fit the polynomial coefficients of ``sincos_rev`` in debye_kernels.hip, and
+print the FP64 Taylor tables of ``sincos_rev_f64`` / ``sin_rev_f64``.
 
 sin(2 pi t) and cos(2 pi t) on |t| <= 1/8 revolution:
 
@@ -92,7 +93,27 @@ def fit_cos():
     return coef, err
 
 
+def taylor_f64():
+    """FP64 tables of sincos_rev_f64 / sin_rev_f64: the Taylor coefficients
+    (-1)^n (2 pi)^m / m! of sin(2 pi t) and cos(2 pi t) - 1, rounded to double,
+    with the truncation bound of each table on its interval."""
+    def c(m):
+        return (-1) ** ((m - 1) // 2 if m % 2 else m // 2) * TWO_PI ** m / mp.factorial(m)
+    quarter_sin = [float(c(m)) for m in range(1, 18, 2)]
+    quarter_cos = [float(c(m)) for m in range(2, 17, 2)]
+    half_sin = [float(c(m)) for m in range(1, 24, 2)]
+    bounds = {"quarter sine (|t| <= 1/8, to t^17)": (mp.pi / 4) ** 19 / mp.factorial(19),
+              "quarter cosine - 1 (|t| <= 1/8, to t^16)": (mp.pi / 4) ** 18 / mp.factorial(18),
+              "half sine (|t| <= 1/4, to t^23)": (mp.pi / 2) ** 25 / mp.factorial(25)}
+    return quarter_sin, quarter_cos, half_sin, bounds
+
+
 if __name__ == "__main__":
+    qs, qc, hs, bounds = taylor_f64()
+    for name, tab in (("kDSinQ", qs), ("kDCosQ", qc), ("kDSinH", hs)):
+        print(f"constexpr double {name}[{len(tab)}] = {{" + ", ".join(repr(v) for v in tab) + "};")
+    for k, v in bounds.items():
+        print(f"// truncation bound, {k}: {mp.nstr(v, 3)}")
     sc, se = fit_sin()
     cc, ce = fit_cos()
     names_s = ("S1_HI", "S1_LO", "S3", "S5", "S7", "S9")
