@@ -81,6 +81,27 @@ def test_nonuniform_q(engine, oracle, precision):
     assert err.max() <= TOL[precision], err.max()
 
 
+@pytest.mark.parametrize("precision,qpl", [("fp32", 64), ("fp32", 128), ("fp64", 32), ("fp64", 64)])
+def test_large_phases_and_coarse_grid(engine, oracle, precision, qpl):
+    """q up to 6 1/A on a coarse grid over an 80 A wide cluster: phases q r / 2 pi
+    up to ~80 revolutions (the seeds' exact reduction of large arguments), and a
+    half step h = dq r / 2 above 1/8 turn for most pairs (the seeds' reduced
+    half-step path instead of the small-h polynomial)."""
+    from pripps_amd._native import tuning
+    pos, kind, q, ffa, ffe, water, w = make_case(600, 23, nq=130, qmax=6.0)
+    pos = pos * (40.0 / np.max(np.linalg.norm(pos - pos.mean(axis=0), axis=1)))
+    case = (pos, kind, q, ffa, ffe, water, w)
+    dq = q[1] - q[0]
+    span = np.max(np.linalg.norm(pos[:, None, :] - pos[None, :, :], axis=2))
+    assert dq * span / (4 * math.pi) > 0.125  # the host cannot prove small h
+    print(f"large phases q_per_lane={qpl} precision={precision}", flush=True)
+    m = build(engine, case, precision, tuning(q_per_lane=qpl))
+    assert m.stats().q_per_lane == qpl
+    err = envelope_error(m.profiles(), oracle_profiles(oracle, case))
+    print(f"large phases q_per_lane={qpl} precision={precision}: max envelope error {err.max():.2e}")
+    assert err.max() <= TOL[precision], (err.max(), np.unravel_index(err.argmax(), err.shape))
+
+
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
 @pytest.mark.parametrize("qpl", [32, 64, 128])
 def test_strip_shapes(engine, oracle, precision, qpl):
