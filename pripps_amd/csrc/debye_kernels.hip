@@ -759,14 +759,16 @@ __global__ void __launch_bounds__(256) debye_tiled_kernel(TiledArgs args) {
     tiled_body<T, Q, UNIFORM, VARIANT, ONEACC>(args);
 }
 
-// FP64, uniform grid, 64 q-points per item, one accumulator array: 64 FP64
-// accumulators (128 VGPRs).  Left to itself the compiler uses 258 registers,
-// one wave per SIMD; capped at two waves it keeps the loops spill-free (one
-// spilled value, reloaded once per item).  One seed per pair serves 64
-// q-points instead of 32, which is where the FP64 path spends its time.
+// FP64 kernels, capped at two waves per SIMD.  Left to itself the compiler
+// gives them 258-288 registers, one wave per SIMD; with the cap the pair loops
+// stay spill-free (the few spilled values live in the per-item set-up and the
+// carry).  Shapes: Q = 64 on uniform grids with one accumulator array (64 FP64
+// accumulators, one seed per pair per 64 q-points), and Q = 32 with two
+// arrays on uniform and non-uniform grids.
+template <int Q, bool UNIFORM, bool ONEACC>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-debye_tiled_f64_q64_kernel(TiledArgs args) {
-    tiled_body<double, 64, true, 3, true>(args);
+debye_tiled_f64_kernel(TiledArgs args) {
+    tiled_body<double, Q, UNIFORM, 3, ONEACC>(args);
 }
 
 // ---------------------------------------------------------------------------
@@ -1163,7 +1165,7 @@ __global__ void combine_kernel(const double *__restrict__ prof, int nq, double v
 // The one-accumulator kernel where it saves a wave's worth of registers: FP32
 // on uniform grids, variants 3 and 4 (the non-uniform FP32 kernel and FP64 at
 // Q = 32 stay at two waves per SIMD either way, so they keep two arrays).
-// FP64 at Q = 64 has its own entry point, debye_tiled_f64_q64_kernel.
+// FP64 has its own entry points, debye_tiled_f64_kernel.
 template <typename T, int Q, bool U, int V>
 constexpr bool kOneAcc = sizeof(T) == 4 && U && V != 1 && V != 2;
 
@@ -1191,12 +1193,15 @@ hipError_t launch_tiled(bool fp64, int Q, bool uniform, const TiledArgs &a, int 
             return hipGetLastError();
         }
     } else {
-        if (Q == 32) return uniform ? launch_tiled_t<double, 32, true>(a, blocks, st)
-                                    : launch_tiled_t<double, 32, false>(a, blocks, st);
-        if (Q == 64 && uniform) {
-            hipLaunchKernelGGL(debye_tiled_f64_q64_kernel, dim3(blocks), dim3(256), 0, st, a);
-            return hipGetLastError();
-        }
+        if (Q == 32 && uniform)
+            hipLaunchKernelGGL((debye_tiled_f64_kernel<32, true, false>), dim3(blocks), dim3(256), 0, st, a);
+        else if (Q == 32)
+            hipLaunchKernelGGL((debye_tiled_f64_kernel<32, false, false>), dim3(blocks), dim3(256), 0, st, a);
+        else if (Q == 64 && uniform)
+            hipLaunchKernelGGL((debye_tiled_f64_kernel<64, true, true>), dim3(blocks), dim3(256), 0, st, a);
+        else
+            return hipErrorInvalidValue;
+        return hipGetLastError();
     }
     return hipErrorInvalidValue;
 }
@@ -1219,10 +1224,11 @@ hipError_t tiled_occupancy(bool fp64, int Q, bool uniform, int variant, int *blo
             return hipOccupancyMaxActiveBlocksPerMultiprocessor(
                 blocks_per_cu, reinterpret_cast<const void *>(&debye_dual_kernel), 256, 0);
     } else {
-        if (Q == 32) return uniform ? occupancy_t<double, 32, true>(blocks_per_cu) : occupancy_t<double, 32, false>(blocks_per_cu);
-        if (Q == 64 && uniform)
-            return hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                blocks_per_cu, reinterpret_cast<const void *>(&debye_tiled_f64_q64_kernel), 256, 0);
+        const void *f = nullptr;
+        if (Q == 32 && uniform) f = reinterpret_cast<const void *>(&debye_tiled_f64_kernel<32, true, false>);
+        else if (Q == 32) f = reinterpret_cast<const void *>(&debye_tiled_f64_kernel<32, false, false>);
+        else if (Q == 64 && uniform) f = reinterpret_cast<const void *>(&debye_tiled_f64_kernel<64, true, true>);
+        if (f) return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, 256, 0);
     }
     return hipErrorInvalidValue;
 }
