@@ -118,14 +118,21 @@ __device__ __forceinline__ Atom<T> load_frac(const PRIPPS_CONST Atom<T> *p, int 
 }
 
 // cos and sin of 2 pi (n . t) for one atom.
+// DIRECT: the host proved |n . t| < 255 revolutions for every vector (|t| <=
+// 1/2 per axis), inside the [-256, 256] domain where v_sin_f32 / v_cos_f32
+// reduce the argument themselves, so the fract is left out.  On negative
+// phases that is also the more accurate form: fract(x) for x < 0 rounds
+// 1 - |x| to the coarser ulp of a number near 1.
+template <bool DIRECT>
 __device__ __forceinline__ void cis_rev(float nx, float ny, float nz, const Atom<float> &t,
                                         float &c, float &s) {
     const float p = __builtin_fmaf(nz, t.z, __builtin_fmaf(ny, t.y, nx * t.x));
-    const float f = __builtin_amdgcn_fractf(p);
+    const float f = DIRECT ? p : __builtin_amdgcn_fractf(p);
     c = __builtin_amdgcn_cosf(f);
     s = __builtin_amdgcn_sinf(f);
 }
 
+template <bool DIRECT>
 __device__ __forceinline__ void cis_rev(double nx, double ny, double nz, const Atom<double> &t,
                                         double &c, double &s) {
     const double p = fma(nz, t.z, fma(ny, t.y, nx * t.x));
@@ -138,7 +145,7 @@ __device__ __forceinline__ void cis_rev(double nx, double ny, double nz, const A
 // step with two chains per sum keep consecutive adds independent, and the
 // next two atoms' scalar loads are issued before the current two are used
 // (every index is clamped to je - 1, so no load leaves the run).
-template <typename T, bool WET>
+template <typename T, bool WET, bool DIRECT>
 __device__ __forceinline__ void class_run(const PRIPPS_CONST Atom<T> *ca, int jb, int je, T nx,
                                           T ny, T nz, T &C, T &S, T &Cw, T &Sw) {
     T c0 = T(0), s0 = T(0), c1 = T(0), s1 = T(0);
@@ -151,8 +158,8 @@ __device__ __forceinline__ void class_run(const PRIPPS_CONST Atom<T> *ca, int jb
         const Atom<T> na = load_frac(ca, min(j + 2, last));
         const Atom<T> nb = load_frac(ca, min(j + 3, last));
         T ca_, sa_, cb_, sb_;
-        cis_rev(nx, ny, nz, a, ca_, sa_);
-        cis_rev(nx, ny, nz, b, cb_, sb_);
+        cis_rev<DIRECT>(nx, ny, nz, a, ca_, sa_);
+        cis_rev<DIRECT>(nx, ny, nz, b, cb_, sb_);
         c0 += ca_;
         s0 += sa_;
         c1 += cb_;
@@ -168,7 +175,7 @@ __device__ __forceinline__ void class_run(const PRIPPS_CONST Atom<T> *ca, int jb
     }
     if (j < je) {  // odd count: `a` holds atom j
         T ca_, sa_;
-        cis_rev(nx, ny, nz, a, ca_, sa_);
+        cis_rev<DIRECT>(nx, ny, nz, a, ca_, sa_);
         c0 += ca_;
         s0 += sa_;
         if constexpr (WET) {
@@ -182,7 +189,7 @@ __device__ __forceinline__ void class_run(const PRIPPS_CONST Atom<T> *ca, int jb
     Sw = sw0 + sw1;
 }
 
-template <typename T>
+template <typename T, bool DIRECT>
 __global__ void __launch_bounds__(256) direct_kernel(DirectArgs args) {
     const int lane = direct_lane();
     const PRIPPS_CONST DirectSegment *segs = (const PRIPPS_CONST DirectSegment *)(args.segs);
@@ -211,8 +218,8 @@ __global__ void __launch_bounds__(256) direct_kernel(DirectArgs args) {
             const int k = segs[sg].kind;
             const int flags = segs[sg].flags;
             T C, S, Cw = T(0), Sw = T(0);
-            if (flags & 1) class_run<T, true>(atoms, j, seg_end, nx, ny, nz, C, S, Cw, Sw);
-            else class_run<T, false>(atoms, j, seg_end, nx, ny, nz, C, S, Cw, Sw);
+            if (flags & 1) class_run<T, true, DIRECT>(atoms, j, seg_end, nx, ny, nz, C, S, Cw, Sw);
+            else class_run<T, false, DIRECT>(atoms, j, seg_end, nx, ny, nz, C, S, Cw, Sw);
             const double dC = double(C), dS = double(S);
             const double fa = args.ffa[(int64_t)k * V + v];
             if (args.self) {
@@ -393,6 +400,7 @@ struct pripps_direct_model {
     int32_t V = 0, G = 0, n_items = 0, n_tiles = 0, blocks = 0;
     int64_t n_atoms = 0;
     bool has_self = false;
+    bool sin_direct = false;  // every |n . t| < 255 revolutions (FP32 kernel)
     std::vector<double> group_q;
     void *dev = nullptr;  // one allocation: inputs, items, partials, amplitudes, outputs
     DirectArgs args{};
@@ -416,9 +424,11 @@ hipError_t enqueue(const pripps_direct_model &m, hipStream_t st, hipEvent_t ev_a
     if (err == hipSuccess && ev_a) err = hipEventRecord(ev_a, st);
     if (err == hipSuccess && m.n_items > 0) {
         if (m.fp64)
-            hipLaunchKernelGGL(direct_kernel<double>, dim3(m.blocks), dim3(256), 0, st, m.args);
+            hipLaunchKernelGGL((direct_kernel<double, false>), dim3(m.blocks), dim3(256), 0, st, m.args);
+        else if (m.sin_direct)
+            hipLaunchKernelGGL((direct_kernel<float, true>), dim3(m.blocks), dim3(256), 0, st, m.args);
         else
-            hipLaunchKernelGGL(direct_kernel<float>, dim3(m.blocks), dim3(256), 0, st, m.args);
+            hipLaunchKernelGGL((direct_kernel<float, false>), dim3(m.blocks), dim3(256), 0, st, m.args);
         err = hipGetLastError();
     }
     if (err == hipSuccess && ev_b) err = hipEventRecord(ev_b, st);
@@ -636,6 +646,15 @@ extern "C" int pripps_direct_model_build(pripps_engine *engine, const pripps_deb
     m->n_tiles = n_tiles;
     m->n_atoms = N;
     m->has_self = self_mode;
+    {
+        // |n . t| <= (|nx| + |ny| + |nz|) / 2 with |t| <= 1/2 per axis
+        int64_t n1 = 0;
+        for (int32_t v = 0; v < V; ++v) {
+            const int32_t *ix = grid->indices + 3 * (int64_t)v;
+            n1 = std::max<int64_t>(n1, std::llabs(ix[0]) + std::llabs(ix[1]) + std::llabs(ix[2]));
+        }
+        m->sin_direct = 0.5 * (double)n1 < 255.0;
+    }
     m->group_q.assign(grid->group_q, grid->group_q + G);
     m->order = order;
     for (int d = 0; d < 3; ++d) m->L[d] = grid->box[d];
@@ -678,10 +697,10 @@ extern "C" int pripps_direct_model_build(pripps_engine *engine, const pripps_deb
                          reinterpret_cast<const int32_t *>(d + o_gvec), m->d_out, G, 0.0, 0.0};
 
     int per_cu = 0;
-    err = fp64 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                     &per_cu, reinterpret_cast<const void *>(&direct_kernel<double>), 256, 0)
-               : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                     &per_cu, reinterpret_cast<const void *>(&direct_kernel<float>), 256, 0);
+    const void *kfn = fp64 ? reinterpret_cast<const void *>(&direct_kernel<double, false>)
+                      : m->sin_direct ? reinterpret_cast<const void *>(&direct_kernel<float, true>)
+                                      : reinterpret_cast<const void *>(&direct_kernel<float, false>);
+    err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, 256, 0);
     if (err != hipSuccess) return abort_build(hip_fail(err, "hipOccupancyMaxActiveBlocksPerMultiprocessor"));
     const int waves_needed = n_items;
     m->blocks = std::max(1, std::min(std::max(per_cu, 1) * acc.n_cu, (waves_needed + 3) / 4));

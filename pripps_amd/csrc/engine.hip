@@ -211,6 +211,7 @@ struct Prepared {
     bool has_e = false, has_h = false;
     bool uniform = false;
     bool small_h = false;   // dq * r_max / (4 pi) <= 1/8
+    bool sin_direct = false;  // q_max * r_max / (2 pi) < 255 revolutions
     double q0 = 0, dq = 0;
     std::vector<double> q;
     std::vector<double> pos;  // [n][4] centred x y z w, sorted by class
@@ -373,6 +374,11 @@ int prepare(const pripps_debye_input *in, Prepared &P) {
     }
     const double rmax = 2.0 * std::sqrt(rmax2) * (1.0 + 1e-6) + 1e-6;
     P.small_h = std::fabs(P.dq) * rmax / (4.0 * M_PI) <= 0.125;
+    double qabs = 0;
+    for (double v : P.q) qabs = std::max(qabs, std::fabs(v));
+    // v_sin_f32 takes |x| <= 256 revolutions; 255 leaves room for the FP32
+    // rounding of q / 2 pi and r
+    P.sin_direct = qabs * rmax / (2.0 * M_PI) < 255.0;
     P.pair_q_terms = (double)n * (double)(n - 1) / 2.0 * (double)nq;
     return PRIPPS_OK;
 }
@@ -680,6 +686,7 @@ static void stage_build(const BuildPlan &B, const std::vector<Prepared> &P, char
         d.K = p.K;
         d.uniform = p.uniform ? 1 : 0;
         d.small_h = p.small_h ? 1 : 0;
+        d.sin_direct = p.sin_direct ? 1 : 0;
         d.n_classes = p.n_classes;
         d.n_strips = B.plain ? 0 : (p.nq + B.sp[s].Q - 1) / B.sp[s].Q;
         d.block = B.sp[s].block;

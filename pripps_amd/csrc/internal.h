@@ -40,7 +40,8 @@ struct DevStruct {
     int32_t n_classes;
     int32_t n_strips;       // ceil(nq / Q)
     int32_t block;          // j atoms summed in FP32 before the FP64 carry
-    int32_t pad_;
+    int32_t sin_direct;     // q_max r_max / 2 pi < 255 revolutions for every pair: the
+                            // hardware sine takes the phase without a fract
     int32_t cls_start[kMaxClasses + 1];
     int8_t cls_kind[kMaxClasses];
     int8_t cls_wet[kMaxClasses];
