@@ -39,6 +39,7 @@
 #include <vector>
 
 #include "internal.h"
+#include "trig_f64.h"
 #include "pripps_direct.h"
 
 namespace pripps {
@@ -136,8 +137,7 @@ template <bool DIRECT>
 __device__ __forceinline__ void cis_rev(double nx, double ny, double nz, const Atom<double> &t,
                                         double &c, double &s) {
     const double p = fma(nz, t.z, fma(ny, t.y, nx * t.x));
-    const double f = p - rint(p);
-    sincospi(2.0 * f, &s, &c);
+    sincos_rev_f64(p, s, c);
 }
 
 // Sums over atoms [jb, je) of one class (je > jb): C = sum cos, S = sum sin
