@@ -744,7 +744,10 @@ debye_tiled_f64_kernel(TiledArgs args) {
 // slot (L2-resident, written only by this wave): in registers they would push
 // the kernel past 256 VGPRs, to one wave per SIMD.
 // ---------------------------------------------------------------------------
-template <bool MASK>
+// TWO = false: the strip's second half lies beyond the grid's last q-point
+// (the last strip of a grid with nq mod 128 in (0, 64]); only the first 64
+// steps are run.
+template <bool MASK, bool TWO>
 __device__ __forceinline__ void dual_block_dry(const Atom<float> *__restrict__ atoms, int jb, int je,
                                                const Atom<float> &ai, int iidx, float lane_take,
                                                const float (&q0p)[2], const float (&q0lo)[2],
@@ -769,11 +772,13 @@ __device__ __forceinline__ void dual_block_dry(const Atom<float> *__restrict__ a
             sv += d;
             d -= z.lam * sv;
         }
+        if constexpr (TWO) {
 #pragma unroll
-        for (int k = 0; k < 64; ++k) {
-            acc_b[k] += sv;
-            sv += d;
-            d -= z.lam * sv;
+            for (int k = 0; k < 64; ++k) {
+                acc_b[k] += sv;
+                sv += d;
+                d -= z.lam * sv;
+            }
         }
     }
 }
@@ -801,6 +806,8 @@ __global__ void __launch_bounds__(256) debye_dual_kernel(TiledArgs args) {
         const bool small_h = sd.small_h != 0;
 
         const int qbase = it.strip * Q;
+        // the second 64-point half holds grid points (else it is left at zero)
+        const bool half2 = qbase + 64 < nq;
         float q0p[2], q0lo[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -864,13 +871,19 @@ __global__ void __launch_bounds__(256) debye_dual_kernel(TiledArgs args) {
             if (!wet) {
 #pragma unroll
                 for (int k = 0; k < 64; ++k) acc_a[k] = acc_b[k] = 0.0f;
-                if (mask) dual_block_dry<true>(atoms, j, jend, ai, iidx, lane_take, q0p, q0lo, hp, hlo, small_h, acc_a, acc_b);
-                else dual_block_dry<false>(atoms, j, jend, ai, iidx, lane_take, q0p, q0lo, hp, hlo, small_h, acc_a, acc_b);
+                if (half2) {
+                    if (mask) dual_block_dry<true, true>(atoms, j, jend, ai, iidx, lane_take, q0p, q0lo, hp, hlo, small_h, acc_a, acc_b);
+                    else dual_block_dry<false, true>(atoms, j, jend, ai, iidx, lane_take, q0p, q0lo, hp, hlo, small_h, acc_a, acc_b);
+                } else {
+                    if (mask) dual_block_dry<true, false>(atoms, j, jend, ai, iidx, lane_take, q0p, q0lo, hp, hlo, small_h, acc_a, acc_b);
+                    else dual_block_dry<false, false>(atoms, j, jend, ai, iidx, lane_take, q0p, q0lo, hp, hlo, small_h, acc_a, acc_b);
+                }
                 carry(0, lk, acc_a, acc_b, false);
-                carry(1, lk, acc_b, acc_a, false);
+                if (half2) carry(1, lk, acc_b, acc_a, false);
             } else {
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
+                    if (h == 1 && !half2) break;
 #pragma unroll
                     for (int k = 0; k < 64; ++k) acc_a[k] = acc_b[k] = 0.0f;
                     if (mask)

@@ -3,7 +3,7 @@
 ``Server.submit`` copies a prepared input and returns a request id at once;
 ``Server.wait`` returns the finished ``NativeModel`` (profiles resident on the
 GPU) and the request's latency from submit to completion.  Requests that
-arrive while a batch runs start in the next batch, on the second of two
+arrive while a batch runs start in the next batch, on another of three
 device lanes, and overlap on the GPU.  A structure's profiles are bitwise the
 same whether it runs alone or batched with others.
 """
