@@ -943,16 +943,23 @@ __device__ void diagonal_and_zero(const PRIPPS_CONST DevStruct &sd, int m, doubl
 // the plan alone.
 __global__ void __launch_bounds__(256) debye_reduce_kernel(FinalizeArgs args) {
     __shared__ double red[256][kProfiles];
-    const int g_strip = blockIdx.y;
-    const int chunk = blockIdx.x;
+    // One block per chunk slot of `stage`: the strip is the last one whose
+    // first slot is <= blockIdx.x (strips without items own no slot).
+    const int slot = blockIdx.x;
+    int lo = 0, hi = args.n_strips;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (args.stage_ptr[mid] <= slot) lo = mid;
+        else hi = mid;
+    }
+    const int g_strip = lo;
+    const int chunk = slot - args.stage_ptr[g_strip];
     const int Q = args.Q;
     const int G = 256 / Q;
     const int ql = threadIdx.x % Q;
     const int g = threadIdx.x / Q;
     const int b = args.csr_ptr[g_strip] + chunk * kFinChunk;
     const int e = min(args.csr_ptr[g_strip + 1], b + kFinChunk);
-    const int nchunks = (args.csr_ptr[g_strip + 1] - args.csr_ptr[g_strip] + kFinChunk - 1) / kFinChunk;
-    if (chunk >= nchunks) return;  // uniform for the whole block
     double acc[kProfiles] = {0, 0, 0, 0, 0, 0};
     if (g < G) {
         for (int idx = b + g; idx < e; idx += G) {
@@ -1207,8 +1214,8 @@ hipError_t tiled_occupancy(bool fp64, int Q, bool uniform, int variant, int *blo
 }
 
 hipError_t launch_finalize(const FinalizeArgs &a, int n_strips_total, hipStream_t st) {
-    if (a.max_chunks > 0) {
-        hipLaunchKernelGGL(debye_reduce_kernel, dim3(a.max_chunks, n_strips_total), dim3(256), 0, st, a);
+    if (a.total_chunks > 0) {
+        hipLaunchKernelGGL(debye_reduce_kernel, dim3(a.total_chunks), dim3(256), 0, st, a);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

@@ -473,7 +473,6 @@ struct LaunchGroup {
     std::vector<int> members;
     std::vector<Item> items;
     std::vector<int32_t> csr_ptr, csr_items, strip_struct, strip_index, stage_ptr;
-    int max_chunks = 0;
     size_t o_items = 0, o_csr_ptr = 0, o_csr_items = 0, o_ss = 0, o_si = 0, o_sp = 0;
     size_t o_counter = 0, o_partials = 0, o_stage = 0;
 };
@@ -570,7 +569,6 @@ static void plan_group(LaunchGroup &g, const std::vector<Prepared> &P,
     for (int64_t x = 0; x < n_strips; ++x) {
         const int nch = (g.csr_ptr[x + 1] - g.csr_ptr[x] + kFinChunk - 1) / kFinChunk;
         g.stage_ptr[x + 1] = g.stage_ptr[x] + nch;
-        g.max_chunks = std::max(g.max_chunks, nch);
     }
     g.strip_struct.resize(n_strips);
     g.strip_index.resize(n_strips);
@@ -743,7 +741,8 @@ static int make_launches(pripps_engine *e, const BuildPlan &B, char *dbase,
         f.strip_index = (const int32_t *)(dbase + g.o_si);
         f.stage_ptr = (const int32_t *)(dbase + g.o_sp);
         f.stage = (double *)(dbase + g.o_stage);
-        f.max_chunks = g.max_chunks;
+        f.n_strips = (int)(g.stage_ptr.size() - 1);
+        f.total_chunks = g.stage_ptr.back();
         f.Q = g.Q;
         launches.push_back(r);
     }

@@ -78,7 +78,8 @@ struct FinalizeArgs {
     const int32_t *strip_index;   // flattened (structure, strip) -> strip
     const int32_t *stage_ptr;     // per (structure, strip): first chunk slot in `stage`
     double *stage;                // [chunks][6][Q] chunk totals
-    int max_chunks;               // largest chunk count of any strip
+    int n_strips;                 // (structure, strip) pairs
+    int total_chunks;             // stage_ptr[n_strips]: one reduce block each
     int Q;
 };
 
