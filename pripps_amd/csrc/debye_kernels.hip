@@ -962,6 +962,7 @@ __global__ void __launch_bounds__(256) debye_reduce_kernel(FinalizeArgs args) {
     const int e = min(args.csr_ptr[g_strip + 1], b + kFinChunk);
     double acc[kProfiles] = {0, 0, 0, 0, 0, 0};
     if (g < G) {
+#pragma unroll 4
         for (int idx = b + g; idx < e; idx += G) {
             const double *p = args.partials + (int64_t)args.csr_items[idx] * (kProfiles * Q);
 #pragma unroll
@@ -995,6 +996,9 @@ __global__ void __launch_bounds__(128) debye_finalize_kernel(FinalizeArgs args) 
     if (m >= sd.nq) return;
     const int nchunks = args.stage_ptr[g_strip + 1] - args.stage_ptr[g_strip];
     double acc[kProfiles] = {0, 0, 0, 0, 0, 0};
+    // unrolled so that several chunks' loads are in flight; the sums keep
+    // chunk order
+#pragma unroll 4
     for (int c = 0; c < nchunks; ++c) {
         const double *p = args.stage + ((int64_t)args.stage_ptr[g_strip] + c) * (kProfiles * Q);
 #pragma unroll
