@@ -77,7 +77,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--workload", default="solvent10k",
-                    choices=("solvent10k", "vacuum1k", "helix60k", "batch512", "served", "direct"))
+                    choices=("solvent10k", "vacuum1k", "helix60k", "batch512", "served", "direct",
+                             "frames"))
+    ap.add_argument("--frames", type=int, default=64, help="frames: trajectory frames")
     ap.add_argument("--cpu-vectors", type=int, default=256,
                     help="direct: q-vectors of the oracle / CPU baseline sample")
     ap.add_argument("--count", type=int, default=512, help="batch512: structures")
@@ -437,6 +439,63 @@ def bench_batch512(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_frames(args):
+    """Trajectory frame averaging (the reference's trajectory_average for the
+    Debye scheme): a solvated globule topology with Gaussian-jittered frames
+    and per-frame weights, through the batched, graph-captured frames path,
+    against building the frames one at a time.  Two topologies: the
+    solvent10k structure (each frame fills the GPU by itself) and a
+    2,000-atom one (many frames per launch)."""
+    import numpy as np
+    from pripps_amd._native import PreparedInput
+    from pripps_amd.synth import solvent10k
+    eng, ff = _engine_and_ff()
+    cases = []
+    for n_atoms, n_frames in ((10000, args.frames), (2000, 8 * args.frames)):
+        wl = solvent10k(n=n_atoms)
+        inp = ff.prepare_workload(wl)
+        rng = np.random.default_rng(7)
+        coords = wl.pos[None, :, :] + rng.normal(0.0, 0.5, size=(n_frames, wl.n_atoms, 3))
+        weights = rng.uniform(0.5, 1.5, size=n_frames)
+        terms = float(wl.pair_q_terms) * n_frames
+        case = {"topology": f"{n_atoms}-atom globule, nq 500, FoXS solvent", "frames": n_frames,
+                "pair_q_terms": terms}
+        m = eng.frames_average(inp, coords, weights, precision="fp32")  # warm-up
+        walls = []
+        for _ in range(max(1, args.steps // 5)):
+            t0 = time.perf_counter()
+            m = eng.frames_average(inp, coords, weights, precision="fp32")
+            avg = m.profiles()
+            walls.append((time.perf_counter() - t0) * 1e3)
+        st = m.stats()
+        wall = float(np.median(walls))
+        case["fp32_frames_path"] = {"wall_ms": wall, "terms_per_s": terms / (wall * 1e-3),
+                                    "frames_per_s": n_frames / (wall * 1e-3),
+                                    "kernel_ms": st.kernel_ms,
+                                    "frames_per_launch": st.frames_per_launch,
+                                    "graph_launches": st.graph_launches}
+        log(f"frames {n_atoms} atoms x {n_frames}: {case['fp32_frames_path']}")
+        # the same frames one build at a time (host arrays in, profiles out)
+        ref = np.zeros_like(avg)
+        t0 = time.perf_counter()
+        for f in range(n_frames):
+            one = PreparedInput(coords[f], inp.kind, inp.q, inp.atom_ff, inp.excluded_ff,
+                                inp.water_ff, inp.weights)
+            ref += weights[f] * eng.build(one, "fp32").profiles()
+        seq = (time.perf_counter() - t0) * 1e3
+        ref /= weights.sum()
+        case["fp32_sequential_builds"] = {"wall_ms": seq, "terms_per_s": terms / (seq * 1e-3)}
+        case["speedup_vs_sequential"] = seq / wall
+        case["max_rel_err_vs_sequential"] = envelope_error(avg, ref)
+        log(f"  sequential: {case['fp32_sequential_builds']}; speedup "
+            f"{case['speedup_vs_sequential']:.2f}x, max envelope difference "
+            f"{case['max_rel_err_vs_sequential']:.2e}")
+        cases.append(case)
+    print(json.dumps({"workload": "frames: Gaussian-jittered frames (N(0, 0.5 A)) of a solvated "
+                                  "globule, weights U[0.5, 1.5], stride 1, frames as host arrays "
+                                  "(upload included)", "cases": cases}), flush=True)
+
+
 def _request_job(job):
     """One request's globule; elements come back as indices into synth.ELEMENTS."""
     import numpy as np
@@ -629,6 +688,7 @@ def bench_direct(args):
 
 
 WORKLOADS = {"vacuum1k": bench_vacuum1k, "helix60k": bench_helix60k, "batch512": bench_batch512,
+             "frames": bench_frames,
              "served": bench_served, "direct": bench_direct}
 
 

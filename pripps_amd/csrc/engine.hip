@@ -16,6 +16,7 @@
 #include <cstring>
 #include <atomic>
 #include <mutex>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -1346,6 +1347,9 @@ extern "C" int pripps_debye_frames_average(pripps_engine *e, const pripps_debye_
         const size_t cap = (size_t)4 << 30;
         C = (int)std::max<size_t>(1, std::min<size_t>((size_t)M, cap / std::max<size_t>(1, per_frame)));
     }
+    // at least four chunks (of >= 16 frames): the host side of every chunk
+    // but the first then overlaps the kernels of the one before
+    C = std::min(C, std::max(16, (M + 3) / 4));
     if (max_frames_per_launch > 0) C = std::min(C, (int)max_frames_per_launch);
 
     std::lock_guard<std::mutex> lock(e->mu);
@@ -1363,6 +1367,7 @@ extern "C" int pripps_debye_frames_average(pripps_engine *e, const pripps_debye_
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     auto cleanup = [&]() {
+        (void)hipStreamSynchronize(st);  // nothing in flight may use what is freed
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
         if (work) (void)hipFree(work);
@@ -1389,39 +1394,87 @@ extern "C" int pripps_debye_frames_average(pripps_engine *e, const pripps_debye_
     int64_t items_per_chunk = 0;
     int graph_launches = 0;
     StructPlan sp0;
-    std::vector<Prepared> P;
-    for (int first = 0; first < M; first += C) {
-        const int cnt = std::min(C, M - first);
-        P.assign(cnt, Prepared{});
-        for (int c = 0; c < cnt; ++c) {
-            if (first + c == 0) {
-                P[0] = p0;
-                continue;
-            }
-            rc = prep_frame(used[first + c], P[c]);
-            if (rc) {
-                cleanup();
-                return rc;
-            }
-            if (P[c].n != p0.n || P[c].nq != p0.nq) {
-                cleanup();
-                return fail(PRIPPS_ERR_VALIDATION, "frame topology changed");
-            }
-        }
+    // Host side of one chunk: its frames prepared (in parallel over frames;
+    // prepare() is a pure function of its input) and planned.  The next
+    // chunk's host side runs while the current chunk's kernels do; only the
+    // pinned staging buffer has to wait for the previous upload.
+    struct ChunkHost {
+        int first = 0, cnt = 0;
+        std::vector<Prepared> P;
         BuildPlan B;
-        plan_build(e, P, precision, tune, (size_t)nq * 8 + (size_t)cnt * 8, B);
-        if (first == 0) {
-            // size the arena and staging once for the largest (first) chunk, so
-            // the captured graph's pointers stay valid for every replay
-            rc = ensure_arena(e, B.total_bytes);
-            if (!rc) rc = ensure_staging(e, B.upload_bytes);
-            if (rc) {
-                cleanup();
-                return rc;
+    };
+    const int host_threads =
+        std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    auto prep_chunk = [&](int first, ChunkHost &H) -> int {
+        H.first = first;
+        H.cnt = std::min(C, M - first);
+        H.P.assign(H.cnt, Prepared{});
+        H.B = BuildPlan{};
+        const int nt = std::max(1, std::min(H.cnt, host_threads));
+        std::vector<int> rcs(H.cnt, PRIPPS_OK);
+        std::vector<std::string> msgs(H.cnt);
+        auto part = [&](int t) {
+            std::vector<double> bx((size_t)n), by((size_t)n), bz((size_t)n);
+            for (int c = t; c < H.cnt; c += nt) {
+                if (first + c == 0) {
+                    H.P[c] = p0;
+                    continue;
+                }
+                const int32_t f = used[first + c];
+                const double *cc = coords + (size_t)f * (size_t)n * 3;
+                for (int64_t i = 0; i < n; ++i) {
+                    bx[i] = cc[3 * i];
+                    by[i] = cc[3 * i + 1];
+                    bz[i] = cc[3 * i + 2];
+                }
+                pripps_debye_input in = *topo;
+                in.x = bx.data();
+                in.y = by.data();
+                in.z = bz.data();
+                rcs[c] = prepare(&in, H.P[c]);
+                if (rcs[c]) {
+                    msgs[c] = "frame " + std::to_string(f) + ": " + g_last_error;  // this thread's
+                } else if (H.P[c].n != p0.n || H.P[c].nq != p0.nq) {
+                    rcs[c] = PRIPPS_ERR_VALIDATION;
+                    msgs[c] = "frame topology changed";
+                }
             }
-            sp0 = B.sp[0];
-            items_per_chunk = B.n_items;
+        };
+        if (nt == 1) {
+            part(0);
+        } else {
+            std::vector<std::thread> pool;
+            for (int t = 0; t < nt; ++t) pool.emplace_back(part, t);
+            for (auto &th : pool) th.join();
         }
+        for (int c = 0; c < H.cnt; ++c)
+            if (rcs[c]) {
+                g_last_error = msgs[c];
+                return rcs[c];
+            }
+        plan_build(e, H.P, precision, tune, (size_t)nq * 8 + (size_t)H.cnt * 8, H.B);
+        return PRIPPS_OK;
+    };
+    ChunkHost cur, next;
+    rc = prep_chunk(0, cur);
+    if (rc) {
+        cleanup();
+        return rc;
+    }
+    // size the arena and staging once for the largest (first) chunk, so the
+    // captured graph's pointers stay valid for every replay
+    rc = ensure_arena(e, cur.B.total_bytes);
+    if (!rc) rc = ensure_staging(e, cur.B.upload_bytes);
+    if (rc) {
+        cleanup();
+        return rc;
+    }
+    sp0 = cur.B.sp[0];
+    items_per_chunk = cur.B.n_items;
+    for (;;) {
+        const BuildPlan &B = cur.B;
+        const std::vector<Prepared> &P = cur.P;
+        const int cnt = cur.cnt, first = cur.first;
         char *dbase = (char *)e->arena;
         char *hbase = (char *)e->staging;
         const double *d_q = (const double *)(dbase + B.o_user);
@@ -1466,10 +1519,21 @@ extern "C" int pripps_debye_frames_average(pripps_engine *e, const pripps_debye_
             FRAMES_CHK(enqueue_all());
         }
         FRAMES_CHK(hipEventRecord(e->ev1, st));
+        // the next chunk's host side overlaps this chunk's kernels
+        const int nfirst = first + cnt;
+        if (nfirst < M) {
+            rc = prep_chunk(nfirst, next);
+            if (rc) {
+                cleanup();
+                return rc;
+            }
+        }
         FRAMES_CHK(hipStreamSynchronize(st));  // staging is rewritten next chunk
         float ms = 0;
         (void)hipEventElapsedTime(&ms, e->ev0, e->ev1);
         kernel_ms += ms;
+        if (nfirst >= M) break;
+        std::swap(cur, next);
     }
     FRAMES_CHK(launch_frames_normalize(acc, nq, wsum, model->d_prof(), st));
     FRAMES_CHK(hipMemcpyAsync(model->d_q(), p0.q.data(), (size_t)nq * 8, hipMemcpyHostToDevice, st));
