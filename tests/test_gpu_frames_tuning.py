@@ -124,6 +124,16 @@ def test_frames_validation(engine, foxs):
         foxs.trajectory_average_from_arrays(ids, frames, weights=[1.0, 1.0, 1.0], **kw)
     with pytest.raises(ValidationError):
         foxs.trajectory_average_from_arrays(ids, frames[:0], **kw)
+    # a bad frame in a later chunk (40 frames run as chunks of 16): its
+    # preparation fails while the first chunk's kernels are in flight
+    many = np.stack([pos + 0.01 * k for k in range(40)])
+    many[25, 7, 1] = np.nan
+    with pytest.raises(ValidationError, match="frame 25.*finite"):
+        foxs.trajectory_average_from_arrays(ids, many, **kw)
+    # the engine is still usable, and the good frames average as before
+    good = foxs.trajectory_average_from_arrays(ids, many[:24], **kw)
+    again = foxs.trajectory_average_from_arrays(ids, many[:24], **kw)
+    assert np.array_equal(good.intensity(1.0, 0.0).total, again.intensity(1.0, 0.0).total)
 
 
 def test_autotune_table_applies_and_keeps_batches_deterministic(foxs):
