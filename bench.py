@@ -400,13 +400,16 @@ def bench_batch512(args):
     inps = [ff.prepare_workload(w) for w in wls]
     terms = float(sum(w.pair_q_terms for w in wls))
     log(f"batch: {len(wls)} structures, {terms:.4e} terms (setup {time.perf_counter() - t:.1f} s)")
+    tb = time.perf_counter()
     models = eng.build_batch(inps, "fp32")
+    first_wall = (time.perf_counter() - tb) * 1e3  # host arrays in, models on the device
     first_ms = models[0].stats().kernel_ms
     ms, pair_ms = _timed(models[0], max(1, args.steps // 4), 1)
     out = {"workload": f"batch{len(wls)}: globules N log-uniform [500,20000], jitter 0.5 A, own q grid "
                        "nq U[100,1000] qmax U[0.3,0.8], solvent",
            "structures": len(wls), "pair_q_terms": terms,
            "fp32_batch": {"ms_per_step": ms, "pair_kernel_ms": pair_ms, "first_build_kernel_ms": first_ms,
+                          "first_build_wall_ms": first_wall,
                           "terms_per_s": terms / (ms * 1e-3)}}
     log(f"fp32 batch: {out['fp32_batch']}")
     m64 = eng.build_batch(inps, "fp64")

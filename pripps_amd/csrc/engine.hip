@@ -784,13 +784,32 @@ static int build_models(pripps_engine *e, int32_t ns, const pripps_debye_input *
     for (int s = 0; s < ns; ++s) out[s] = nullptr;
     if (ns == 0) return PRIPPS_OK;
 
+    // host preparation, in parallel over structures for a batch (prepare()
+    // is a pure function of its input; errors are reported for the first
+    // failing structure in input order)
     std::vector<Prepared> P(ns);
-    for (int s = 0; s < ns; ++s) {
-        int rc = prepare(&inputs[s], P[s]);
-        if (rc) {
-            if (ns > 1) g_last_error = "structure " + std::to_string(s) + ": " + g_last_error;
-            return rc;
+    {
+        const int nt = std::max(1, std::min(ns / 2, std::min(8, (int)std::thread::hardware_concurrency())));
+        std::vector<int> rcs(ns, PRIPPS_OK);
+        std::vector<std::string> msgs(ns);
+        auto part = [&](int t) {
+            for (int s = t; s < ns; s += nt) {
+                rcs[s] = prepare(&inputs[s], P[s]);
+                if (rcs[s]) msgs[s] = g_last_error;  // this thread's message
+            }
+        };
+        if (nt == 1) {
+            part(0);
+        } else {
+            std::vector<std::thread> pool;
+            for (int t = 0; t < nt; ++t) pool.emplace_back(part, t);
+            for (auto &th : pool) th.join();
         }
+        for (int s = 0; s < ns; ++s)
+            if (rcs[s]) {
+                g_last_error = ns > 1 ? "structure " + std::to_string(s) + ": " + msgs[s] : msgs[s];
+                return rcs[s];
+            }
     }
 
     std::lock_guard<std::mutex> lock(e->mu);
