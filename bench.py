@@ -97,7 +97,7 @@ def main():
                     help="default run: requests in its short served stream (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build "
+        print("bench.py: the engine runs on one GPU and has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
