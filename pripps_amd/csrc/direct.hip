@@ -57,6 +57,7 @@ namespace {
 
 constexpr int kDirVals = 7;     // Re A, Im A, Re E, Im E, Re H, Im H, self
 constexpr int kDirBlock = 256;  // atoms summed in the working precision per carry
+constexpr int kAtomPad = 8;   // zero atoms after the last one (FP32 prefetch)
 
 struct DirectItem {
     int32_t tile;    // vectors [64 tile, 64 tile + 64)
@@ -145,9 +146,73 @@ __device__ __forceinline__ void cis_rev(double nx, double ny, double nz, const A
 // step with two chains per sum keep consecutive adds independent, and the
 // next two atoms' scalar loads are issued before the current two are used
 // (every index is clamped to je - 1, so no load leaves the run).
+// Four consecutive FP32 atoms: one 64-byte scalar load.
+struct alignas(16) AtomQuad {
+    Atom<float> a[4];
+};
+
+__device__ __forceinline__ AtomQuad load_quad(const PRIPPS_CONST Atom<float> *ca, int j) {
+    const PRIPPS_CONST float *f = (const PRIPPS_CONST float *)(ca + j);
+    AtomQuad r;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        r.a[t].x = f[4 * t];
+        r.a[t].y = f[4 * t + 1];
+        r.a[t].z = f[4 * t + 2];
+        r.a[t].w = f[4 * t + 3];
+    }
+    return r;
+}
+
+// FP32 run over atoms [jb, je): four atoms per step from one scalar load,
+// the next four prefetched.  The scalar unit is shared by the CU's four
+// SIMDs; with one load and index update per atom (the FP64 form below) its
+// instructions per term matched the vector ones and it became the limit.
+// Loads past je read the following atoms or the zero padding after the last
+// atom (kAtomPad >= 8), and are not used.
+template <bool WET, bool DIRECT>
+__device__ __forceinline__ void class_run_f32(const PRIPPS_CONST Atom<float> *ca, int jb, int je,
+                                              float nx, float ny, float nz, float &C, float &S,
+                                              float &Cw, float &Sw) {
+    float c0 = 0.f, s0 = 0.f, c1 = 0.f, s1 = 0.f;
+    float cw0 = 0.f, sw0 = 0.f, cw1 = 0.f, sw1 = 0.f;
+    auto one = [&](const Atom<float> &a, float &c, float &s, float &cw, float &sw) {
+        float ct, st;
+        cis_rev<DIRECT>(nx, ny, nz, a, ct, st);
+        c += ct;
+        s += st;
+        if constexpr (WET) {
+            cw = fmaf(a.w, ct, cw);
+            sw = fmaf(a.w, st, sw);
+        }
+    };
+    AtomQuad q = load_quad(ca, jb);
+    int j = jb;
+    for (; j + 3 < je; j += 4) {
+        const AtomQuad nq = load_quad(ca, j + 4);
+        one(q.a[0], c0, s0, cw0, sw0);
+        one(q.a[1], c1, s1, cw1, sw1);
+        one(q.a[2], c0, s0, cw0, sw0);
+        one(q.a[3], c1, s1, cw1, sw1);
+        q = nq;
+    }
+    // 0-3 atoms left, in q
+    if (j < je) one(q.a[0], c0, s0, cw0, sw0);
+    if (j + 1 < je) one(q.a[1], c1, s1, cw1, sw1);
+    if (j + 2 < je) one(q.a[2], c0, s0, cw0, sw0);
+    C = c0 + c1;
+    S = s0 + s1;
+    Cw = cw0 + cw1;
+    Sw = sw0 + sw1;
+}
+
 template <typename T, bool WET, bool DIRECT>
 __device__ __forceinline__ void class_run(const PRIPPS_CONST Atom<T> *ca, int jb, int je, T nx,
                                           T ny, T nz, T &C, T &S, T &Cw, T &Sw) {
+    if constexpr (sizeof(T) == 4) {
+        class_run_f32<WET, DIRECT>(ca, jb, je, nx, ny, nz, C, S, Cw, Sw);
+        return;
+    }
     T c0 = T(0), s0 = T(0), c1 = T(0), s1 = T(0);
     T cw0 = T(0), sw0 = T(0), cw1 = T(0), sw1 = T(0);
     const int last = je - 1;
@@ -598,7 +663,9 @@ extern "C" int pripps_direct_model_build(pripps_engine *engine, const pripps_deb
         off = align_up(off + bytes);
         return o;
     };
-    const size_t o_atoms = take(std::max<int64_t>(N, 1) * 4 * esz);
+    // kAtomPad zero atoms after the last one: the FP32 kernel prefetches up
+    // to that far past the end of a run (the values are never used)
+    const size_t o_atoms = take((size_t)(N + kAtomPad) * 4 * esz);
     const size_t o_nvec = take((size_t)std::max(V_pad, 1) * 4 * esz);
     const size_t o_ffa = take(std::max<size_t>(KV, 1) * sizeof(double));
     const size_t o_ffe = take(in->excluded_ff ? KV * sizeof(double) : 0);
