@@ -355,8 +355,13 @@ def bench_helix60k(args):
                      "build_kernel_ms": m.stats().kernel_ms}
         log(f"helix60k {prec}: {out[prec]}")
     perq = _per_q_error(profs["fp32"], profs["fp64"])
+    worst = np.argsort(perq)[::-1][:5]
     out["fp32_vs_fp64_per_q"] = {"max": float(perq.max()), "median": float(np.median(perq)),
-                                 "q_at_max": float(wl.q[int(perq.argmax())])}
+                                 "p90": float(np.percentile(perq, 90)),
+                                 "p99": float(np.percentile(perq, 99)),
+                                 "q_at_max": float(wl.q[int(perq.argmax())]),
+                                 "worst": [[float(wl.q[k]), float(perq[k])] for k in worst],
+                                 "values": [float(f"{v:.3g}") for v in perq]}
     sel = np.array([150, 500, 900])
     threads = min(16, os.cpu_count() or 1)
     tc = time.perf_counter()
