@@ -346,19 +346,32 @@ __global__ void __launch_bounds__(256) direct_kernel(DirectArgs args) {
     }
 }
 
-// amp[v][7] = sum over the items of v's tile, in span order.
+// amp[v][7] = sum over the items of v's tile.  One block per 64-vector
+// tile: four groups of 64 threads take the tile's items g, g + 4, ... (span
+// order), and the four group sums are added in group order, so the result is
+// a function of the plan alone.
 __global__ void __launch_bounds__(256) direct_finalize_kernel(FinalizeDirectArgs a) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= a.V) return;
-    const int tile = v / 64, lane = v % 64;
+    __shared__ double red[4][kDirVals][64];
+    const int tile = blockIdx.x;
+    const int lane = threadIdx.x % 64, g = threadIdx.x / 64;
     double acc[kDirVals] = {0, 0, 0, 0, 0, 0, 0};
-    for (int p = a.tile_ptr[tile]; p < a.tile_ptr[tile + 1]; ++p) {
+    const int b = a.tile_ptr[tile], e = a.tile_ptr[tile + 1];
+#pragma unroll 4
+    for (int p = b + g; p < e; p += 4) {
         const double *src = a.partials + (int64_t)a.tile_items[p] * (kDirVals * 64) + lane;
 #pragma unroll
         for (int t = 0; t < kDirVals; ++t) acc[t] += src[t * 64];
     }
 #pragma unroll
-    for (int t = 0; t < kDirVals; ++t) a.amp[(int64_t)v * kDirVals + t] = acc[t];
+    for (int t = 0; t < kDirVals; ++t) red[g][t][lane] = acc[t];
+    __syncthreads();
+    const int v = tile * 64 + lane;
+    if (g == 0 && v < a.V) {
+#pragma unroll
+        for (int t = 0; t < kDirVals; ++t)
+            a.amp[(int64_t)v * kDirVals + t] =
+                ((red[0][t][lane] + red[1][t][lane]) + red[2][t][lane]) + red[3][t][lane];
+    }
 }
 
 // Group means, in the reference's operation order per vector:
@@ -521,7 +534,7 @@ hipError_t enqueue(const pripps_direct_model &m, hipStream_t st, hipEvent_t ev_a
     }
     if (err == hipSuccess && ev_b) err = hipEventRecord(ev_b, st);
     if (err == hipSuccess && m.V > 0) {
-        hipLaunchKernelGGL(direct_finalize_kernel, dim3((m.V + 255) / 256), dim3(256), 0, st,
+        hipLaunchKernelGGL(direct_finalize_kernel, dim3(m.n_tiles), dim3(256), 0, st,
                            m.fargs);
         err = hipGetLastError();
     }
@@ -648,7 +661,7 @@ extern "C" int pripps_direct_model_build(pripps_engine *engine, const pripps_deb
     int64_t span = 0;
     std::vector<DirectItem> items;
     if (N > 0 && V > 0) {
-        const int64_t target_items = 64LL * acc.n_cu;  // several items per wave slot
+        const int64_t target_items = 256LL * acc.n_cu;  // several items per wave slot
         span = std::max<int64_t>(256, (N * n_tiles + target_items - 1) / target_items);
         if (self_mode) span = std::max<int64_t>(1, (span + M - 1) / M) * M;
         span = std::min<int64_t>(span, N);
