@@ -40,7 +40,7 @@ FLOP_PER_TERM_SOLVENT = 14.0
 #   executed_flop_per_term: rocprofv3 TOTAL_32_OPS ((2 FMA + ADD + MUL + TRANS +
 #          INT32) x 64 per dispatch) / pair-q terms.
 COUNTERS_SOLVENT10K = {"q_per_lane": 128, "span": 256, "work_items": 12928,
-                       "bytes": (41444.25 * 2.0 + 198713.5 * 1.0) * 1024,
+                       "bytes": (2472.4 * 2.0 + 78036.0 * 1.0) * 1024,
                        "executed_flop_per_term": 1.2833e11 / 2.49975e10}
 PAIRS = ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))
 

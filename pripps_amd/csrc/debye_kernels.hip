@@ -740,9 +740,11 @@ debye_tiled_f64_kernel(TiledArgs args) {
 // The first half of a dry pair and both halves of a hydrated pair see the
 // same seeds, recurrence and carry as the 64-point kernel; the second half of
 // a dry pair continues the recurrence for 64 more steps instead of starting
-// from a fresh seed.  The FP64 profile sums live in the item's own partial
-// slot (L2-resident, written only by this wave): in registers they would push
-// the kernel past 256 VGPRs, to one wave per SIMD.
+// from a fresh seed.  The FP64 profile sums live in LDS (6 KB per wave) and
+// go to the item's partial slot once at the end: in registers they would push
+// the kernel past 256 VGPRs, to one wave per SIMD, and in the partial slot
+// their read-modify-write per chunk cost HBM traffic (288 against 85 MB per
+// launch on a 10,000-atom structure).
 // ---------------------------------------------------------------------------
 // TWO = false: the strip's second half lies beyond the grid's last q-point
 // (the last strip of a grid with nq mod 128 in (0, 64]); only the first 64
@@ -786,6 +788,10 @@ __device__ __forceinline__ void dual_block_dry(const Atom<float> *__restrict__ a
 __global__ void __launch_bounds__(256) debye_dual_kernel(TiledArgs args) {
     constexpr int Q = 128;
     const int lane = __lane_id();
+    // the wave's FP64 profile sums: [profile][q-local], each lane only ever
+    // touches its own two columns (lane, 64 + lane), so no barrier is needed
+    __shared__ double sums[4][kProfiles * Q];
+    double *const lsum = sums[threadIdx.x / 64];
     for (;;) {
         int w = 0;
         if (lane == 0) w = atomicAdd(args.counter, 1);
@@ -823,14 +829,14 @@ __global__ void __launch_bounds__(256) debye_dual_kernel(TiledArgs args) {
         for (int k = 0; k < 64; ++k) qv_unused[k] = ql_unused[k] = 0.0f;
 
         // Lane owns q = qbase + 64 h + lane of each half h.  The FP64 profile
-        // accumulators live in the item's own partial slot (written only by
-        // this wave, L2-resident): a carry adds to it once per chunk, which
-        // keeps them and the form factors out of the register file.
+        // accumulators live in LDS (6 KB per wave): a carry adds to them once
+        // per chunk, which keeps them and the form factors out of the register
+        // file, and the item's partial slot is written once at the end.
         double *out = args.partials + (int64_t)w * (kProfiles * Q);
 #pragma unroll
         for (int h = 0; h < 2; ++h)
 #pragma unroll
-            for (int p = 0; p < kProfiles; ++p) out[p * Q + 64 * h + lane] = 0.0;
+            for (int p = 0; p < kProfiles; ++p) lsum[p * Q + 64 * h + lane] = 0.0;
 
         auto carry = [&](int h, int lk, const float (&a0)[64], const float (&aw)[64], bool wet) {
             float r0[1], ri[1], rw[1], riw[1];
@@ -849,7 +855,7 @@ __global__ void __launch_bounds__(256) debye_dual_kernel(TiledArgs args) {
             const double el = sd.ffe ? sd.ffe[lk * nq + m] : 0.0;
             const double S = double(r0[0]), Si = double(ri[0]);
             const double Sw = double(rw[0]), Siw = double(riw[0]);
-            double *o = out + 64 * h + lane;
+            double *o = lsum + 64 * h + lane;
             o[0 * Q] += 2.0 * ak * al * S;
             o[1 * Q] += 2.0 * ek * el * S;
             o[2 * Q] += 2.0 * wq * wq * Siw;
@@ -895,7 +901,10 @@ __global__ void __launch_bounds__(256) debye_dual_kernel(TiledArgs args) {
             }
             j = jend;
         }
-
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int p = 0; p < kProfiles; ++p) out[p * Q + 64 * h + lane] = lsum[p * Q + 64 * h + lane];
     }
 }
 
