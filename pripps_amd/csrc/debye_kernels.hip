@@ -991,28 +991,40 @@ __global__ void __launch_bounds__(256) debye_reduce_kernel(FinalizeArgs args) {
     }
 }
 
-// Stage 2: one thread per q-point of a strip sums the chunk totals in chunk
-// order, divides by q and adds the diagonal (q = 0: closed form).
-__global__ void __launch_bounds__(128) debye_finalize_kernel(FinalizeArgs args) {
+// Stage 2: one block per strip.  The block's kFinThreads threads form
+// kFinThreads / Q groups of Q; group g sums the strip's chunk totals
+// g, g + G, ... in chunk order, and the group sums are added in group order
+// (the result is a function of the plan alone).  Then the q-point's thread
+// divides by q and adds the diagonal (q = 0: closed form).
+constexpr int kFinThreads = 1024;
+
+__global__ void __launch_bounds__(kFinThreads) debye_finalize_kernel(FinalizeArgs args) {
+    __shared__ double red[kFinThreads][kProfiles];
     const int g_strip = blockIdx.x;
-    const int ql = threadIdx.x;
     const int Q = args.Q;
-    if (ql >= Q) return;
+    const int G = kFinThreads / Q;
+    const int ql = threadIdx.x % Q, g = threadIdx.x / Q;
     const int s = args.strip_struct[g_strip];
     const int strip = args.strip_index[g_strip];
     const PRIPPS_CONST DevStruct &sd = as_const(args.structs)[s];
     const int m = strip * Q + ql;
-    if (m >= sd.nq) return;
     const int nchunks = args.stage_ptr[g_strip + 1] - args.stage_ptr[g_strip];
     double acc[kProfiles] = {0, 0, 0, 0, 0, 0};
-    // unrolled so that several chunks' loads are in flight; the sums keep
-    // chunk order
+    if (m < sd.nq) {
 #pragma unroll 4
-    for (int c = 0; c < nchunks; ++c) {
-        const double *p = args.stage + ((int64_t)args.stage_ptr[g_strip] + c) * (kProfiles * Q);
+        for (int c = g; c < nchunks; c += G) {
+            const double *p = args.stage + ((int64_t)args.stage_ptr[g_strip] + c) * (kProfiles * Q);
 #pragma unroll
-        for (int k = 0; k < kProfiles; ++k) acc[k] += p[k * Q + ql];
+            for (int k = 0; k < kProfiles; ++k) acc[k] += p[k * Q + ql];
+        }
     }
+#pragma unroll
+    for (int k = 0; k < kProfiles; ++k) red[threadIdx.x][k] = acc[k];
+    __syncthreads();
+    if (g != 0 || m >= sd.nq) return;
+    for (int gg = 1; gg < G; ++gg)
+#pragma unroll
+        for (int k = 0; k < kProfiles; ++k) acc[k] += red[gg * Q + ql][k];
     double diag[kProfiles], zero[kProfiles];
     diagonal_and_zero(sd, m, diag, zero);
     const double q = sd.q[m];
@@ -1232,7 +1244,7 @@ hipError_t launch_finalize(const FinalizeArgs &a, int n_strips_total, hipStream_
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(debye_finalize_kernel, dim3(n_strips_total), dim3(128), 0, st, a);
+    hipLaunchKernelGGL(debye_finalize_kernel, dim3(n_strips_total), dim3(kFinThreads), 0, st, a);
     return hipGetLastError();
 }
 
