@@ -81,6 +81,26 @@ def test_nonuniform_q(engine, oracle, precision):
     assert err.max() <= TOL[precision], err.max()
 
 
+@pytest.mark.parametrize("precision,n,nq,want_q", [
+    # FP32: 128-point strips from 2,048 atoms when nq > 64, else 64 (32 for nq <= 32)
+    ("fp32", 2047, 65, 64), ("fp32", 2048, 64, 64), ("fp32", 2048, 65, 128),
+    ("fp32", 2048, 129, 128), ("fp32", 2048, 192, 128), ("fp32", 2048, 193, 128),
+    ("fp32", 2048, 32, 32),
+    # FP64: 64-point strips from 2,048 atoms when nq > 32, else 32
+    ("fp64", 2047, 129, 32), ("fp64", 2048, 129, 64), ("fp64", 2048, 32, 32),
+])
+def test_default_plan_boundaries(engine, oracle, precision, n, nq, want_q):
+    """The default strip length at its boundaries in N and nq, including grids
+    whose last 128-point strip has an empty second half (nq = 129, 192) or a
+    partly filled one (65, 193)."""
+    case = make_case(n, 40 + nq, nq=nq)
+    print(f"plan boundary n={n} nq={nq} precision={precision}", flush=True)
+    m = build(engine, case, precision)
+    assert m.stats().q_per_lane == want_q
+    err = envelope_error(m.profiles(), oracle_profiles(oracle, case))
+    assert err.max() <= TOL[precision], (err.max(), np.unravel_index(err.argmax(), err.shape))
+
+
 @pytest.mark.parametrize("precision,qpl", [("fp32", 64), ("fp32", 128), ("fp64", 32), ("fp64", 64)])
 def test_large_phases_and_coarse_grid(engine, oracle, precision, qpl):
     """q up to 6 1/A on a coarse grid over an 80 A wide cluster: phases q r / 2 pi
