@@ -297,3 +297,38 @@ def test_direct_trajectory_average(engine, oracle, tmp_path, precision):
         p.direct_trajectory_average(str(xyz), frames[:, :-1], box_sides=60.0, pmax=2)
     with pytest.raises(ValueError):
         p.direct_trajectory_average(str(xyz), frames, box_sides=60.0, pmax=2, stride=0)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("grid_positions", [True, False])
+def test_large_miller_indices(engine, oracle, tmp_path, precision, grid_positions):
+    """Miller p up to 200 in a 64 A box: |n . t| reaches 300 revolutions, beyond
+    the 255 the hardware sine takes unreduced, so the FP32 kernel reduces each
+    phase with fract first (the other FP32 form of direct_kernel).  With atoms
+    on a 1/1024 grid of the box every product and sum of n . t is exact in FP32
+    and both precisions meet their usual tolerance; at generic positions the
+    FP32 phase carries its rounding at 300 revolutions (~5e-5 relative in the
+    intensity), held to the reference's own f32 bar of 1e-3 (src/gpu.rs:666-700)."""
+    import pripps_amd
+    yaml = tmp_path / "xy.yaml"
+    yaml.write_text("X: { formfactor: !Constant 1.0 }\nY: { formfactor: !Constant 0.5 }\n")
+    L, pmax, n = 64.0, 200, 64
+    rng = np.random.default_rng(11)
+    if grid_positions:
+        pos = rng.integers(-640, 1400, size=(n, 3)) * (L / 1024.0)
+    else:
+        pos = rng.uniform(-40.0, 90.0, size=(n, 3))
+    ids = np.arange(n) % 2
+    p = pripps_amd.Pripps(str(yaml), engine=engine)
+    m = p.direct_model_from_arrays(pos, ids, pmax=pmax, box_sides=L, precision=precision)
+    from pripps_amd.direct import QGrid
+    idx, qn, group, gq = QGrid.miller(pmax).groups((L,) * 3)
+    assert 0.5 * np.abs(idx).sum(axis=1).max() >= 255.0
+    ffa = np.array([np.full(len(idx), 1.0), np.full(len(idx), 0.5)])
+    rows = oracle.direct_rows(pos, ids.astype(np.int32), (L,) * 3, idx, ffa)
+    err = _rel(m.intensity(1.0, 0.0).total,
+               _grouped(rows[:, 0] ** 2 + rows[:, 1] ** 2, group, len(gq)))
+    kind = "1/1024 grid" if grid_positions else "generic"
+    print(f"direct Miller p<=200 ({kind} positions) {precision}: max rel err {err:.3e}")
+    tol = TOL[precision] if (grid_positions or precision == "fp64") else 1e-3
+    assert err <= tol
