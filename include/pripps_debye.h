@@ -291,9 +291,17 @@ int pripps_server_create(pripps_engine *engine, const pripps_server_config *conf
 int pripps_server_submit(pripps_server *server, const pripps_debye_input *input,
                          int64_t *request_id);
 /* Wait up to timeout_ms (negative = forever) for a request.  On success the
- * caller owns *model.  PRIPPS_ERR_NOT_READY when the wait timed out. */
+ * caller owns *model.  PRIPPS_ERR_NOT_READY when the wait timed out.  A
+ * request's status is its own: when a batch fails, its requests are built
+ * again one by one, so one request with invalid input (a non-finite
+ * coordinate, say) fails alone.  One call at a time may wait for a request; a
+ * second concurrent wait for the same id returns PRIPPS_ERR_USAGE.  After a
+ * successful or failed (not timed-out) wait the id is unknown. */
 int pripps_server_wait(pripps_server *server, int64_t request_id, int32_t timeout_ms,
                        pripps_debye_model **model, double *latency_ms);
+/* Stops the lanes after the queued requests are built, then frees the
+ * requests and models nobody waited for.  Call it after every wait has
+ * returned. */
 int pripps_server_destroy(pripps_server *server);
 
 #ifdef __cplusplus

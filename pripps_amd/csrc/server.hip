@@ -52,6 +52,7 @@ struct Request {
     uint64_t bytes = 0;
     // completion
     bool done = false;
+    bool waiting = false;  // a pripps_server_wait call holds this request
     int status = PRIPPS_OK;
     std::string error;
     pripps_debye_model *model = nullptr;
@@ -131,21 +132,34 @@ struct pripps_server {
                 }
                 in_use += batch_bytes;
             }
-            std::vector<pripps_debye_input> ins(batch.size());
-            for (size_t k = 0; k < batch.size(); ++k) ins[k] = batch[k]->in;
-            std::vector<pripps_debye_model *> out(batch.size(), nullptr);
-            const int rc = engine_build(lane, (int32_t)batch.size(), ins.data(), cfg.precision,
-                                        nullptr, out.data());
-            const std::string err = rc ? get_last_error() : std::string();
+            const size_t nb = batch.size();
+            std::vector<pripps_debye_input> ins(nb);
+            for (size_t k = 0; k < nb; ++k) ins[k] = batch[k]->in;
+            std::vector<pripps_debye_model *> out(nb, nullptr);
+            std::vector<int> rcs(nb, PRIPPS_OK);
+            std::vector<std::string> errs(nb);
+            const int rc = engine_build(lane, (int32_t)nb, ins.data(), cfg.precision, nullptr,
+                                        out.data());
+            if (rc != PRIPPS_OK) {
+                // A batch fails as a whole (one request with non-finite input
+                // fails its validation).  Build each request alone so that
+                // only the requests that fail by themselves report an error.
+                const std::string err = get_last_error();
+                for (size_t k = 0; k < nb; ++k) {
+                    out[k] = nullptr;
+                    rcs[k] = nb == 1 ? rc : engine_build(lane, 1, &ins[k], cfg.precision, nullptr, &out[k]);
+                    if (rcs[k] != PRIPPS_OK) errs[k] = nb == 1 ? err : get_last_error();
+                }
+            }
             const auto now = Clock::now();
             {
                 std::lock_guard<std::mutex> lk(mu);
                 in_use -= batch_bytes;
-                for (size_t k = 0; k < batch.size(); ++k) {
+                for (size_t k = 0; k < nb; ++k) {
                     Request *r = batch[k];
                     r->done = true;
-                    r->status = rc;
-                    r->error = err;
+                    r->status = rcs[k];
+                    r->error = errs[k];
                     r->model = out[k];
                     r->latency_ms = std::chrono::duration<double, std::milli>(now - r->t_submit).count();
                     // inputs are no longer needed
@@ -261,9 +275,17 @@ extern "C" int pripps_server_wait(pripps_server *s, int64_t id, int32_t timeout_
         return PRIPPS_ERR_USAGE;
     }
     Request *r = it->second;
+    if (r->waiting) {
+        // the other waiter frees the request when it returns
+        set_last_error("pripps_server_wait: request " + std::to_string(id) +
+                       " is already being waited for");
+        return PRIPPS_ERR_USAGE;
+    }
+    r->waiting = true;
     auto ready = [&] { return r->done; };
     if (timeout_ms < 0) s->cv_done.wait(lk, ready);
     else if (!s->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready)) {
+        r->waiting = false;
         set_last_error("request " + std::to_string(id) + " still running");
         return PRIPPS_ERR_NOT_READY;
     }

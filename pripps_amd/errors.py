@@ -4,10 +4,11 @@ The variants and their Python exception bases follow the reference:
 ``Error`` (``src/error.rs:16-74``) and the pyo3 mapping ``into_py_err``
 (``pripps-py/src/lib.rs:805-818``): File/Io -> ``IOError``; Yaml,
 UnknownAtom, UndefinedFormFactor, QOutOfRange, Validation, Usage ->
-``ValueError``; Internal -> ``RuntimeError``.  Two engine-only variants are
-added for the device: ``DeviceError`` (a HIP call failed) and
+``ValueError``; Internal -> ``RuntimeError``.  Engine-only variants
+for the device: ``DeviceError`` (a HIP call failed) and
 ``OutOfDeviceMemoryError`` (admission or allocation refused), both
-``RuntimeError``s.
+``RuntimeError``s.  ``NotReadyError`` (a ``TimeoutError``) reports a served
+request that was still running when a wait with a time limit ran out.
 
 The integer codes are the C ABI's ``pripps_status`` values.
 """
@@ -72,8 +73,14 @@ class OutOfDeviceMemoryError(PrippsError, RuntimeError):
     code = 11
 
 
+class NotReadyError(PrippsError, TimeoutError):
+    """A served request was still running when the wait timed out."""
+    code = 12
+
+
 _BY_CODE = {c.code: c for c in (FileError, IoError, YamlError, ValidationError, UsageError,
-                                InternalError, DeviceError, OutOfDeviceMemoryError)}
+                                InternalError, DeviceError, OutOfDeviceMemoryError,
+                                NotReadyError)}
 
 
 def from_status(code: int, message: str) -> PrippsError:

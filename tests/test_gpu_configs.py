@@ -123,3 +123,41 @@ def test_server_alone_equals_in_mix_and_admission(engine, foxs_engine):
         rid = srv.submit(big)
         with pytest.raises(OutOfDeviceMemoryError):
             srv.wait(rid)
+
+
+def test_server_bad_request_fails_alone(engine, foxs_engine):
+    """A request whose coordinates hold a NaN fails with a validation error;
+    the requests batched with it are built again one by one and are the same
+    as built alone.  A second concurrent wait for one id is refused."""
+    from pripps_amd.errors import NotReadyError, UsageError, ValidationError
+    from pripps_amd.server import Server
+    from pripps_amd.synth import Workload, globule, linear_q, radius_for
+    rng = np.random.default_rng(9)
+    inps = []
+    for s in range(8):
+        n = int(rng.integers(300, 2000))
+        names, pos, w = globule(n, radius_for(n), 2000 + s)
+        wl = Workload(f"r{s}", names, pos, linear_q(0.0, 0.5, int(rng.integers(100, 400))), w)
+        inps.append(foxs_engine.prepare_workload(wl))
+    bad = 5
+    inps[bad].y[17] = np.nan
+    with Server(engine, "fp32") as srv:
+        ids = [srv.submit(i) for i in inps]
+        got = {}
+        for k, rid in enumerate(ids):
+            if k == bad:
+                with pytest.raises(ValidationError, match="finite"):
+                    srv.wait(rid)
+            else:
+                got[k] = srv.wait(rid)[0]
+        # one waiter at a time: a wait that timed out leaves the id waitable
+        slow = srv.submit(inps[0])
+        try:
+            m, _ = srv.wait(slow, timeout_ms=0)  # finished already
+        except NotReadyError:
+            m, _ = srv.wait(slow)
+        assert np.array_equal(m.profiles(), got[0].profiles())
+        with pytest.raises(UsageError, match="unknown request"):
+            srv.wait(slow)
+    for k, m in got.items():
+        assert np.array_equal(m.profiles(), engine.build(inps[k], "fp32").profiles())

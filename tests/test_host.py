@@ -166,6 +166,7 @@ def test_error_mapping():
         assert isinstance(E.from_status(code, "x"), ValueError)
     assert isinstance(E.from_status(9, "x"), RuntimeError)
     assert isinstance(E.from_status(11, "x"), E.OutOfDeviceMemoryError)
+    assert isinstance(E.from_status(12, "x"), TimeoutError)
     assert str(E.from_status(7, "bad")) == "bad"
 
 
