@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <atomic>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <string>
@@ -415,13 +416,17 @@ int valid_q(int precision, bool uniform, int Q) {
 
 // Uniform grids: the longest strip whose accumulators fit, so that one
 // recurrence seed serves more q-points -- FP32 128 (the dual kernel), FP64 64
-// -- unless the grid is short or the structure small (below ~2,000 atoms
-// longer strips leave too few items to fill the GPU).  Non-uniform grids
-// take a direct sine per term and keep 32.
+// -- unless the grid is short or the structure small.  A small structure
+// alone is latency-bound and its longer strips leave too few items to fill
+// the GPU: FP32 takes 128 from 3e8 pair-q terms (N^2 nq / 2) on, where the
+// two plans measured even (1,200 atoms x 500 q; 64 was 16 % faster at 1,200 x
+// 200, 128 16 % faster at 1,200 x 800).  Non-uniform grids take a direct sine
+// per term and keep 32.
 int default_q(int precision, bool uniform, int nq, int n_atoms) {
     if (!uniform || nq <= 32) return 32;
     if (precision == PRIPPS_FP64) return n_atoms < 2048 ? 32 : 64;
-    return (nq <= 64 || n_atoms < 2048) ? 64 : 128;
+    const double terms = 0.5 * (double)n_atoms * (double)n_atoms * (double)nq;
+    return (nq <= 64 || terms < 3e8) ? 64 : 128;
 }
 
 // j atoms per work item: about 16 items per wave slot of the GPU when the
@@ -527,28 +532,34 @@ static StructPlan resolve_plan(pripps_engine *e, const pripps_tuning *tune, int 
 
 static void plan_group(LaunchGroup &g, const std::vector<Prepared> &P,
                        const std::vector<StructPlan> &sp) {
-    for (int s : g.members) {
-        const Prepared &p = P[s];
-        const int nst = (p.nq + g.Q - 1) / g.Q;
-        const int span = sp[s].span;
-        for (const int4 &t : p.tiles)
-            for (int32_t j0 = t.x; j0 < p.n; j0 += span) {
-                const int32_t j1 = std::min<int32_t>(p.n, j0 + span);
-                for (int st = 0; st < nst; ++st) g.items.push_back(Item{s, t.x, t.y, t.z, j0, j1, st, 0});
-            }
-    }
-    // largest first, by a stable counting sort on the span length (a
-    // structure's items keep one relative order in any batch)
-    {
-        int maxlen = 0;
-        for (const Item &it : g.items) maxlen = std::max(maxlen, it.j1 - it.j0);
-        std::vector<int64_t> pos(maxlen + 2, 0);
-        for (const Item &it : g.items) pos[maxlen - (it.j1 - it.j0) + 1]++;
-        for (int x = 0; x <= maxlen; ++x) pos[x + 1] += pos[x];
-        std::vector<Item> sorted(g.items.size());
-        for (const Item &it : g.items) sorted[pos[maxlen - (it.j1 - it.j0)]++] = it;
-        g.items.swap(sorted);
-    }
+    // Queue order: largest span first, and within one span length the order
+    // of generation (a stable counting sort: a structure's items keep one
+    // relative order in any batch).  The items are generated twice, once to
+    // count them per span length and once to write each straight to its
+    // place, so no unsorted copy of the list is made.
+    auto each_item = [&](auto &&emit) {
+        for (int s : g.members) {
+            const Prepared &p = P[s];
+            const int nst = (p.nq + g.Q - 1) / g.Q;
+            const int span = sp[s].span;
+            for (const int4 &t : p.tiles)
+                for (int32_t j0 = t.x; j0 < p.n; j0 += span)
+                    emit(s, t, j0, std::min<int32_t>(p.n, j0 + span), nst);
+        }
+    };
+    int maxlen = 0;
+    for (int s : g.members) maxlen = std::max(maxlen, sp[s].span);
+    std::vector<int64_t> pos(maxlen + 2, 0);
+    each_item([&](int, const int4 &, int32_t j0, int32_t j1, int nst) {
+        pos[maxlen - (j1 - j0) + 1] += nst;
+    });
+    for (int x = 0; x <= maxlen; ++x) pos[x + 1] += pos[x];
+    g.items.resize((size_t)pos[maxlen + 1]);
+    Item *items = g.items.data();
+    each_item([&](int s, const int4 &t, int32_t j0, int32_t j1, int nst) {
+        int64_t &w = pos[maxlen - (j1 - j0)];
+        for (int st = 0; st < nst; ++st) items[w++] = Item{s, t.x, t.y, t.z, j0, j1, st, 0};
+    });
     // CSR of items per (member, strip), in queue order
     const int nm = (int)g.members.size();
     std::vector<int64_t> base(nm + 1, 0);
@@ -648,10 +659,11 @@ static void plan_build(pripps_engine *e, const std::vector<Prepared> &P, int pre
 }
 
 // Fill the pinned staging buffer for plan B: atoms (converted), tables,
-// structure records (pointing at the arena, q[s] and prof[s]) and work lists.
+// structure records (pointing at the arena, q[s] and prof[s]) and, unless the
+// buffer holds them from an earlier staging of the same plan, the work lists.
 static void stage_build(const BuildPlan &B, const std::vector<Prepared> &P, char *hbase,
                         char *dbase, const std::vector<const double *> &d_q,
-                        const std::vector<double *> &d_prof) {
+                        const std::vector<double *> &d_prof, bool work_lists = true) {
     std::vector<DevStruct> ds(B.ns);
     for (int s = 0; s < B.ns; ++s) {
         const Prepared &p = P[s];
@@ -697,6 +709,7 @@ static void stage_build(const BuildPlan &B, const std::vector<Prepared> &P, char
         }
     }
     std::memcpy(hbase + B.o_structs, ds.data(), B.ns * sizeof(DevStruct));
+    if (!work_lists) return;
     auto put = [&](size_t off, const void *src, size_t bytes) {
         if (bytes) std::memcpy(hbase + off, src, bytes);
     };
@@ -1416,19 +1429,23 @@ extern "C" int pripps_debye_frames_average(pripps_engine *e, const pripps_debye_
     // Host side of one chunk: its frames prepared (in parallel over frames;
     // prepare() is a pure function of its input) and planned.  The next
     // chunk's host side runs while the current chunk's kernels do; only the
-    // pinned staging buffer has to wait for the previous upload.
+    // pinned staging buffer has to wait for the previous upload.  A plan
+    // depends on the topology, the q grid and the frame count only, so every
+    // full chunk shares the first one's (the captured graph relies on that
+    // too); only the remainder chunk is planned anew.
     struct ChunkHost {
         int first = 0, cnt = 0;
         std::vector<Prepared> P;
-        BuildPlan B;
+        std::shared_ptr<const BuildPlan> B;
     };
+    std::shared_ptr<const BuildPlan> full_plan;
     const int host_threads =
         std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
     auto prep_chunk = [&](int first, ChunkHost &H) -> int {
         H.first = first;
         H.cnt = std::min(C, M - first);
         H.P.assign(H.cnt, Prepared{});
-        H.B = BuildPlan{};
+        H.B.reset();
         const int nt = std::max(1, std::min(H.cnt, host_threads));
         std::vector<int> rcs(H.cnt, PRIPPS_OK);
         std::vector<std::string> msgs(H.cnt);
@@ -1471,7 +1488,14 @@ extern "C" int pripps_debye_frames_average(pripps_engine *e, const pripps_debye_
                 g_last_error = msgs[c];
                 return rcs[c];
             }
-        plan_build(e, H.P, precision, tune, (size_t)nq * 8 + (size_t)H.cnt * 8, H.B);
+        if (H.cnt == C && full_plan) {
+            H.B = full_plan;
+        } else {
+            auto b = std::make_shared<BuildPlan>();
+            plan_build(e, H.P, precision, tune, (size_t)nq * 8 + (size_t)H.cnt * 8, *b);
+            H.B = b;
+            if (H.cnt == C) full_plan = H.B;
+        }
         return PRIPPS_OK;
     };
     ChunkHost cur, next;
@@ -1482,16 +1506,17 @@ extern "C" int pripps_debye_frames_average(pripps_engine *e, const pripps_debye_
     }
     // size the arena and staging once for the largest (first) chunk, so the
     // captured graph's pointers stay valid for every replay
-    rc = ensure_arena(e, cur.B.total_bytes);
-    if (!rc) rc = ensure_staging(e, cur.B.upload_bytes);
+    rc = ensure_arena(e, cur.B->total_bytes);
+    if (!rc) rc = ensure_staging(e, cur.B->upload_bytes);
     if (rc) {
         cleanup();
         return rc;
     }
-    sp0 = cur.B.sp[0];
-    items_per_chunk = cur.B.n_items;
+    sp0 = cur.B->sp[0];
+    items_per_chunk = cur.B->n_items;
+    const BuildPlan *staged = nullptr;  // the plan whose work lists the staging buffer holds
     for (;;) {
-        const BuildPlan &B = cur.B;
+        const BuildPlan &B = *cur.B;
         const std::vector<Prepared> &P = cur.P;
         const int cnt = cur.cnt, first = cur.first;
         char *dbase = (char *)e->arena;
@@ -1501,7 +1526,8 @@ extern "C" int pripps_debye_frames_average(pripps_engine *e, const pripps_debye_
         std::vector<const double *> dq(cnt, d_q);
         std::vector<double *> dprof(cnt);
         for (int c = 0; c < cnt; ++c) dprof[c] = work + (size_t)c * prof_n;
-        stage_build(B, P, hbase, dbase, dq, dprof);
+        stage_build(B, P, hbase, dbase, dq, dprof, staged != &B);
+        staged = &B;
         std::memcpy(hbase + B.o_user, p0.q.data(), (size_t)nq * 8);
         double *hw = (double *)(hbase + B.o_user) + nq;
         for (int c = 0; c < cnt; ++c) hw[c] = w ? w[used[first + c]] : 1.0;

@@ -82,10 +82,11 @@ def test_nonuniform_q(engine, oracle, precision):
 
 
 @pytest.mark.parametrize("precision,n,nq,want_q", [
-    # FP32: 128-point strips from 2,048 atoms when nq > 64, else 64 (32 for nq <= 32)
-    ("fp32", 2047, 65, 64), ("fp32", 2048, 64, 64), ("fp32", 2048, 65, 128),
-    ("fp32", 2048, 129, 128), ("fp32", 2048, 192, 128), ("fp32", 2048, 193, 128),
-    ("fp32", 2048, 32, 32),
+    # FP32: 128-point strips from 3e8 pair-q terms (N^2 nq / 2) when nq > 64,
+    # else 64 (32 for nq <= 32)
+    ("fp32", 2000, 149, 64), ("fp32", 2000, 150, 128), ("fp32", 3100, 64, 64),
+    ("fp32", 3100, 65, 128), ("fp32", 2200, 129, 128), ("fp32", 2000, 192, 128),
+    ("fp32", 2000, 193, 128), ("fp32", 2048, 32, 32),
     # FP64: 64-point strips from 2,048 atoms when nq > 32, else 32
     ("fp64", 2047, 129, 32), ("fp64", 2048, 129, 64), ("fp64", 2048, 32, 32),
 ])
