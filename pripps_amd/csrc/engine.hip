@@ -1612,6 +1612,21 @@ int engine_build(pripps_engine *e, int32_t n, const pripps_debye_input *inputs, 
 }
 void set_last_error(const std::string &msg) { g_last_error = msg; }
 const std::string &get_last_error() { return g_last_error; }
+// used by the serving loop: a model built on a lane is handed to another
+// engine of the same device, so that its queries (intensity, profiles, fit)
+// run on that engine's stream under its lock instead of queueing behind the
+// lane's next batch.  The build has completed (build_models synchronises the
+// lane's stream), so the profiles are visible to every stream of the device.
+// The model's kernels are no longer the new engine's last build: recompute
+// refuses it.
+void model_rehome(pripps_debye_model *m, pripps_engine *to) {
+    if (!m || !to || m->engine == to) return;
+    to->refs.fetch_add(1);
+    pripps_engine *from = m->engine;
+    m->engine = to;
+    m->build_serial = 0;
+    engine_release(from);
+}
 
 // used by the Direct transform (direct.hip): the engine's device, stream and
 // submission lock; a model keeps its engine alive through retain/release.

@@ -10,7 +10,9 @@
 // alongside it (the second kernel's waves take CUs as the first one's drain).
 // Finished models keep their profiles on the device and are handed to the
 // caller by pripps_server_wait together with the request's latency (submit to
-// completion).
+// completion).  They move from their lane to the engine the server was
+// created on, so that the caller's intensity() and fit() calls on them do not
+// wait for the lane's next batch.
 //
 // The reference has no serving layer; this loop wraps the build that replaces
 // debye::build_model (src/debye.rs:78-136).
@@ -35,6 +37,9 @@ int engine_build(pripps_engine *e, int32_t n, const pripps_debye_input *inputs, 
                  const pripps_tuning *tuning, pripps_debye_model **out);
 void set_last_error(const std::string &msg);
 const std::string &get_last_error();
+void model_rehome(pripps_debye_model *m, pripps_engine *to);
+void engine_retain(pripps_engine *e);
+void engine_unref(pripps_engine *e);
 }  // namespace pripps
 
 using namespace pripps;
@@ -81,6 +86,7 @@ uint64_t estimate_bytes(const pripps_debye_input &in) {
 
 struct pripps_server {
     int device = 0;
+    pripps_engine *base = nullptr;  // the caller's engine: finished models answer queries on it
     pripps_server_config cfg{};
     std::vector<pripps_engine *> lanes;
     std::vector<std::thread> workers;
@@ -151,6 +157,8 @@ struct pripps_server {
                     if (rcs[k] != PRIPPS_OK) errs[k] = nb == 1 ? err : get_last_error();
                 }
             }
+            for (size_t k = 0; k < nb; ++k)
+                if (out[k]) model_rehome(out[k], base);
             const auto now = Clock::now();
             {
                 std::lock_guard<std::mutex> lk(mu);
@@ -207,6 +215,8 @@ extern "C" int pripps_server_create(pripps_engine *engine, const pripps_server_c
         }
         s->lanes.push_back(lane);
     }
+    engine_retain(engine);  // released by pripps_server_destroy
+    s->base = engine;
     for (auto *lane : s->lanes) s->workers.emplace_back([s, lane] { s->worker(lane); });
     *out = s;
     return PRIPPS_OK;
@@ -313,6 +323,7 @@ extern "C" int pripps_server_destroy(pripps_server *s) {
         delete kv.second;
     }
     for (auto *l : s->lanes) pripps_engine_destroy(l);
+    engine_unref(s->base);
     delete s;
     return PRIPPS_OK;
 }
