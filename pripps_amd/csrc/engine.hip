@@ -27,12 +27,15 @@
 namespace pripps {
 
 // fit (fit_kernels.hip)
-int device_fit(hipStream_t st, const double *d_prof, const double *d_q, int nq, const double *q,
-               const double *inten, const double *sigma, int64_t n, int method, int c1_points,
-               int c2_points, const double *c1b, const double *c2b, pripps_fit_result *out,
-               std::string *err);
-int device_spline_total(hipStream_t st, const double *d_prof, const double *d_q, int nq, double vs,
-                        double cd, const double *qe, int64_t n, double *out, std::string *err);
+size_t device_fit_doubles(int nq, int64_t n, int method, int c1_points, int c2_points);
+int device_fit(hipStream_t st, double *buf, const double *d_prof, const double *d_q, int nq,
+               const double *q, const double *inten, const double *sigma, int64_t n, int method,
+               int c1_points, int c2_points, const double *c1b, const double *c2b,
+               pripps_fit_result *out, std::string *err);
+size_t device_spline_total_doubles(int nq, int64_t n);
+int device_spline_total(hipStream_t st, double *buf, const double *d_prof, const double *d_q,
+                        int nq, double vs, double cd, const double *qe, int64_t n, double *out,
+                        std::string *err);
 
 }  // namespace pripps
 
@@ -100,6 +103,10 @@ struct pripps_engine {
     size_t arena_size = 0;
     void *staging = nullptr;  // pinned (hipHostMalloc)
     size_t staging_size = 0;
+    // fit scratch, grow-only: a hipFree per fit call would wait for every
+    // stream of the device (the serving lanes' pair kernels included)
+    double *fit_buf = nullptr;
+    size_t fit_doubles = 0;
     // launches of the most recent build, valid while the arena holds them
     uint64_t build_serial = 0;
     bool last_plain = false;
@@ -176,10 +183,27 @@ extern "C" int pripps_engine_create(int device, pripps_engine **out) {
     return PRIPPS_OK;
 }
 
+// Caller holds e->mu with e's device current.
+static int ensure_fit(pripps_engine *e, size_t doubles) {
+    if (doubles <= e->fit_doubles) return 0;
+    if (e->fit_buf) {
+        hipError_t err = hipFree(e->fit_buf);
+        e->fit_buf = nullptr;
+        e->fit_doubles = 0;
+        if (err != hipSuccess) return hip_fail(err, "hipFree(fit scratch)");
+    }
+    const size_t sz = std::max(doubles, (size_t)1 << 16);
+    hipError_t err = hipMalloc((void **)&e->fit_buf, sz * sizeof(double));
+    if (err != hipSuccess) return hip_fail(err, "hipMalloc(fit scratch)");
+    e->fit_doubles = sz;
+    return 0;
+}
+
 static void engine_release(pripps_engine *e) {
     if (e->refs.fetch_sub(1) != 1) return;
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->fit_buf) (void)hipFree(e->fit_buf);
     if (e->arena) (void)hipFree(e->arena);
     if (e->staging) (void)hipHostFree(e->staging);
     if (e->ev0) (void)hipEventDestroy(e->ev0);
@@ -1028,9 +1052,11 @@ extern "C" int pripps_debye_model_fit(const pripps_debye_model *m, const double 
     pripps_engine *e = m->engine;
     std::lock_guard<std::mutex> lock(e->mu);
     HIPCHK(hipSetDevice(e->device));
+    int rc = ensure_fit(e, device_fit_doubles(m->nq, n, method, c1_points, c2_points));
+    if (rc) return rc;
     std::string err;
-    int rc = device_fit(e->stream, m->d_prof(), m->d_q(), m->nq, q, inten, sigma, n, method,
-                        c1_points, c2_points, c1b, c2b, out, &err);
+    rc = device_fit(e->stream, e->fit_buf, m->d_prof(), m->d_q(), m->nq, q, inten, sigma, n,
+                    method, c1_points, c2_points, c1b, c2b, out, &err);
     if (rc) return fail(rc, err);
     return PRIPPS_OK;
 }
@@ -1098,8 +1124,11 @@ extern "C" int pripps_debye_model_fit_curve(const pripps_debye_model *m, double 
     pripps_engine *e = m->engine;
     std::lock_guard<std::mutex> lock(e->mu);
     HIPCHK(hipSetDevice(e->device));
+    int rc = ensure_fit(e, device_spline_total_doubles(m->nq, n));
+    if (rc) return rc;
     std::string err;
-    int rc = device_spline_total(e->stream, m->d_prof(), m->d_q(), m->nq, vs, cd, q, n, out, &err);
+    rc = device_spline_total(e->stream, e->fit_buf, m->d_prof(), m->d_q(), m->nq, vs, cd, q, n,
+                             out, &err);
     if (rc) return fail(rc, err);
     return PRIPPS_OK;
 }

@@ -261,9 +261,16 @@ __global__ void fit_nm_kernel(ExpView v, double lo1, double hi1, double lo2, dou
     res[4] = 0.0;
 }
 
-int device_fit(hipStream_t st, const double *d_prof, const double *d_q, int nq, const double *q,
-               const double *inten, const double *sigma, int64_t n, int method, int c1p, int c2p,
-               const double *c1b, const double *c2b, pripps_fit_result *out, std::string *err) {
+// Device scratch (doubles) that device_fit needs; the caller owns it.
+size_t device_fit_doubles(int nq, int64_t n, int method, int c1p, int c2p) {
+    const size_t grid = method == PRIPPS_FIT_GRID ? (size_t)c1p * c2p : 0;
+    return (size_t)3 * n + (size_t)12 * nq + (size_t)6 * n + grid + 8;
+}
+
+int device_fit(hipStream_t st, double *buf, const double *d_prof, const double *d_q, int nq,
+               const double *q, const double *inten, const double *sigma, int64_t n, int method,
+               int c1p, int c2p, const double *c1b, const double *c2b, pripps_fit_result *out,
+               std::string *err) {
     const int ne = (int)n;
     std::vector<double> host((size_t)3 * ne);
     for (int e = 0; e < ne; ++e) {
@@ -272,13 +279,7 @@ int device_fit(hipStream_t st, const double *d_prof, const double *d_q, int nq, 
         host[2 * ne + e] = sigma ? sigma[e] : 1.0;
     }
     const size_t grid = method == PRIPPS_FIT_GRID ? (size_t)c1p * c2p : 0;
-    const size_t doubles = host.size() + (size_t)12 * nq + (size_t)6 * ne + grid + 8;
-    double *buf = nullptr;
-    hipError_t e = hipMalloc((void **)&buf, doubles * sizeof(double));
-    if (e != hipSuccess) {
-        *err = std::string("hipMalloc(fit): ") + hipGetErrorString(e);
-        return e == hipErrorOutOfMemory ? PRIPPS_ERR_OUT_OF_MEMORY : PRIPPS_ERR_DEVICE;
-    }
+    hipError_t e = hipSuccess;
     double *d_exp = buf;
     double *d_y2 = d_exp + host.size();
     double *d_u = d_y2 + (size_t)6 * nq;
@@ -306,8 +307,6 @@ int device_fit(hipStream_t st, const double *d_prof, const double *d_q, int nq, 
         if ((e = hipMemcpyAsync(res, d_res, sizeof(res), hipMemcpyDeviceToHost, st)) != hipSuccess) break;
         e = hipStreamSynchronize(st);
     } while (0);
-    hipError_t ef = hipFree(buf);
-    if (e == hipSuccess) e = ef;
     if (e != hipSuccess) {
         *err = std::string("fit: ") + hipGetErrorString(e);
         return PRIPPS_ERR_DEVICE;
@@ -326,16 +325,16 @@ int device_fit(hipStream_t st, const double *d_prof, const double *d_q, int nq, 
 // The theory curve at experimental q as write_fit_csv evaluates it
 // (src/fit.rs:548-560): the natural spline of the combined total I(q) over
 // the model's grid, evaluated at qe.  FP64 on the device.
-int device_spline_total(hipStream_t st, const double *d_prof, const double *d_q, int nq, double vs,
-                        double cd, const double *qe, int64_t n, double *out, std::string *err) {
+// Device scratch (doubles) that device_spline_total needs; the caller owns it.
+size_t device_spline_total_doubles(int nq, int64_t n) {
+    return (size_t)4 * nq + (size_t)2 * nq + (size_t)2 * n + 8;
+}
+
+int device_spline_total(hipStream_t st, double *buf, const double *d_prof, const double *d_q,
+                        int nq, double vs, double cd, const double *qe, int64_t n, double *out,
+                        std::string *err) {
     const int ne = (int)n;
-    double *buf = nullptr;
-    const size_t doubles = (size_t)4 * nq + (size_t)2 * nq + (size_t)2 * ne + 8;
-    hipError_t e = hipMalloc((void **)&buf, doubles * sizeof(double));
-    if (e != hipSuccess) {
-        *err = std::string("hipMalloc(fit curve): ") + hipGetErrorString(e);
-        return e == hipErrorOutOfMemory ? PRIPPS_ERR_OUT_OF_MEMORY : PRIPPS_ERR_DEVICE;
-    }
+    hipError_t e = hipSuccess;
     double *d_tot = buf;             // [4][nq]: combine output, row 0 = total
     double *d_y2 = d_tot + (size_t)4 * nq;
     double *d_u = d_y2 + nq;
@@ -353,8 +352,6 @@ int device_spline_total(hipStream_t st, const double *d_prof, const double *d_q,
         if ((e = hipMemcpyAsync(out, d_out, (size_t)ne * 8, hipMemcpyDeviceToHost, st)) != hipSuccess) break;
         e = hipStreamSynchronize(st);
     } while (0);
-    hipError_t ef = hipFree(buf);
-    if (e == hipSuccess) e = ef;
     if (e != hipSuccess) {
         *err = std::string("fit curve: ") + hipGetErrorString(e);
         return PRIPPS_ERR_DEVICE;
