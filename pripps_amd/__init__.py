@@ -101,6 +101,17 @@ class FitResult:
     scale: float
 
 
+def linear_q_values(qmin: float, qmax: float, nq: int) -> np.ndarray:
+    """The linear q grid of ``QSampler::linear`` (src/q_sampler.rs:24-29):
+    q_i = qmin + (qmax - qmin) * i / max(nq - 1, 1) in that operation order,
+    in FP64 (not qmin + i * dq, which rounds differently)."""
+    nq = int(nq)
+    if nq < 0:
+        raise ValidationError("nq must be >= 0")
+    denom = float(max(nq - 1, 1))
+    return np.array([qmin + (qmax - qmin) * i / denom for i in range(nq)], dtype=np.float64)
+
+
 def _fit_method(method) -> int:
     if method is None:
         return _native.FIT_NELDER_MEAD
@@ -319,10 +330,7 @@ class Pripps:
                                 tuning=None, probe=1.8, spacing=3.10516, thickness=3.0,
                                 excluded_spacing=1.0) -> DebyeModel:
         if q_values is None:
-            nq = int(nq)
-            if nq < 0:
-                raise ValidationError("nq must be >= 0")
-            q = np.array([qmin + (qmax - qmin) * i / float(max(nq - 1, 1)) for i in range(nq)])
+            q = linear_q_values(qmin, qmax, nq)
         else:
             q = np.ascontiguousarray(q_values, dtype=np.float64)
         inp = self.prepare_input(pos, ids, q, excluded=excluded, hydration=hydration,
@@ -375,10 +383,7 @@ class Pripps:
         if isinstance(weights, (str, os.PathLike)):
             weights = read_weights(weights)
         if q_values is None:
-            nq = int(nq)
-            if nq < 0:
-                raise ValidationError("nq must be >= 0")
-            q = np.array([qmin + (qmax - qmin) * i / float(max(nq - 1, 1)) for i in range(nq)])
+            q = linear_q_values(qmin, qmax, nq)
         else:
             q = np.ascontiguousarray(q_values, dtype=np.float64)
         pos0 = f[0] if len(f) else np.zeros((len(ids), 3))
@@ -620,7 +625,7 @@ def read_weights(path) -> np.ndarray:
 
 __all__ = [
     "Pripps", "DebyeModel", "Intensity", "FitResult", "Excluded", "Hydration", "FitMethod",
-    "Engine", "read_weights", "ExpData", "read_exp_data", "write_intensity_csv", "write_fit_csv",
+    "Engine", "read_weights", "linear_q_values", "ExpData", "read_exp_data", "write_intensity_csv", "write_fit_csv",
     "DEFAULT_C1_BOUNDS", "DEFAULT_C2_BOUNDS", "PrippsError", "FileError", "YamlError",
     "UnknownAtomError", "UndefinedFormFactorError", "QOutOfRangeError", "ValidationError",
     "UsageError", "InternalError", "DeviceError", "OutOfDeviceMemoryError",

@@ -13,7 +13,9 @@
 //   The phase in revolutions is n . t with t = x / L reduced to [-1/2, 1/2) on
 //   the host (exact: n is an integer vector, so only t mod 1 matters), then
 //   cos and sin of 2 pi fract(n . t).  FP32 mode uses the hardware sine and
-//   cosine of a revolution argument; FP64 mode uses sincospi.
+//   cosine of a revolution argument; FP64 mode uses the FP64 sine and cosine
+//   of a revolution argument in trig_f64.h (exact reduction to the nearest
+//   quarter turn, Taylor polynomials on |t| <= 1/8 turn).
 //
 // Accumulation
 //   Atoms are sorted by class (kind, hydrated) inside each molecule, and the

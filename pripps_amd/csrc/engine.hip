@@ -66,6 +66,80 @@ extern "C" const char *pripps_last_error(void) { return g_last_error.c_str(); }
 extern "C" const char *pripps_version(void) { return "pripps_amd 0.1.0 (gfx950)"; }
 
 // ---------------------------------------------------------------------------
+// device block pool for model buffers
+// ---------------------------------------------------------------------------
+// A model holds [6 nq profiles | nq q | 4 nq combine output] doubles for its
+// whole life.  Those blocks come from a per-device pool of power-of-two size
+// classes (4 KiB and up) carved from 4 MiB slabs; a freed block goes back to
+// its class's free list.  Slabs stay with the process, so releasing a model
+// never calls hipFree, which would wait for every stream of the device -- in
+// the serving loop, for the other lanes' pair kernels.  What the pool holds
+// is the high-water mark of live models per class.
+namespace {
+constexpr int kPoolMinLog = 12;
+constexpr int kPoolMaxLog = 46;
+constexpr size_t kPoolSlab = (size_t)4 << 20;
+
+struct DevicePool {
+    std::mutex mu;
+    std::vector<void *> free_list[kPoolMaxLog + 1];
+    size_t held = 0;
+};
+
+DevicePool &device_pool(int device) {
+    static std::mutex mu;
+    static std::vector<std::unique_ptr<DevicePool>> pools;
+    std::lock_guard<std::mutex> lk(mu);
+    if ((int)pools.size() <= device) pools.resize((size_t)device + 1);
+    if (!pools[device]) pools[device].reset(new DevicePool());
+    return *pools[device];
+}
+
+int pool_class(size_t bytes) {
+    int c = kPoolMinLog;
+    while (c < kPoolMaxLog && ((size_t)1 << c) < bytes) ++c;
+    return c;
+}
+}  // namespace
+
+namespace pripps {
+// The caller has `device` current.
+hipError_t pool_alloc(int device, size_t bytes, void **out) {
+    *out = nullptr;
+    const int c = pool_class(bytes);
+    if (((size_t)1 << c) < bytes) return hipErrorOutOfMemory;
+    DevicePool &P = device_pool(device);
+    std::lock_guard<std::mutex> lk(P.mu);
+    auto &fl = P.free_list[c];
+    if (fl.empty()) {
+        const size_t bs = (size_t)1 << c;
+        const size_t slab = std::max(bs, kPoolSlab);
+        void *base = nullptr;
+        hipError_t err = hipMalloc(&base, slab);
+        if (err != hipSuccess) return err;
+        P.held += slab;
+        for (size_t off = slab; off > 0; off -= bs) fl.push_back((char *)base + off - bs);
+    }
+    *out = fl.back();
+    fl.pop_back();
+    return hipSuccess;
+}
+
+void pool_free(int device, void *p, size_t bytes) {
+    if (!p) return;
+    DevicePool &P = device_pool(device);
+    std::lock_guard<std::mutex> lk(P.mu);
+    P.free_list[pool_class(bytes)].push_back(p);
+}
+
+size_t pool_held_bytes(int device) {
+    DevicePool &P = device_pool(device);
+    std::lock_guard<std::mutex> lk(P.mu);
+    return P.held;
+}
+}  // namespace pripps
+
+// ---------------------------------------------------------------------------
 // engine
 // ---------------------------------------------------------------------------
 namespace pripps {
@@ -101,6 +175,8 @@ struct pripps_engine {
     std::mutex mu;  // serialises every submission on `stream` and the arenas
     void *arena = nullptr;
     size_t arena_size = 0;
+    std::atomic<uint64_t> arena_pub{0};  // arena_size, readable without `mu` (serving admission)
+    size_t arena_prev = 0;               // size of the last arena allocated (growth step)
     void *staging = nullptr;  // pinned (hipHostMalloc)
     size_t staging_size = 0;
     // fit scratch, grow-only: a hipFree per fit call would wait for every
@@ -120,7 +196,8 @@ struct pripps_debye_model {
     pripps_engine *engine = nullptr;
     int32_t nq = 0;
     std::vector<double> q;
-    double *dev = nullptr;  // [6 nq profiles | nq q | 4 nq combine output]
+    double *dev = nullptr;  // [6 nq profiles | nq q | 4 nq combine output], a pool block
+    int device = 0;         // the pool `dev` belongs to
     pripps_build_stats stats{};
     uint64_t build_serial = 0;
     double *d_prof() const { return dev; }
@@ -130,18 +207,47 @@ struct pripps_debye_model {
 
 static void engine_release(pripps_engine *e);
 
+static size_t model_bytes(int32_t nq) { return std::max<size_t>(1, (size_t)11 * nq) * sizeof(double); }
+
+// The model's device block, from the pool of its engine's device (current).
+static hipError_t model_alloc(pripps_debye_model *m) {
+    m->device = m->engine->device;
+    void *p = nullptr;
+    hipError_t err = pool_alloc(m->device, model_bytes(m->nq), &p);
+    m->dev = (double *)p;
+    return err;
+}
+
+static void model_release_block(pripps_debye_model *m) {
+    if (m->dev) pool_free(m->device, m->dev, model_bytes(m->nq));
+    m->dev = nullptr;
+}
+
+// Caller holds e->mu with e's device current.
 static int ensure_arena(pripps_engine *e, size_t bytes) {
     if (bytes <= e->arena_size) return 0;
     if (e->arena) {
         hipError_t err = hipFree(e->arena);
         e->arena = nullptr;
         e->arena_size = 0;
+        e->arena_pub = 0;
         if (err != hipSuccess) return hip_fail(err, "hipFree(arena)");
     }
-    size_t sz = std::max(bytes, (size_t)64 << 20);
+    // a quarter of headroom over the previous size, in 2 MiB steps, so that a
+    // slowly growing stream of builds reallocates (and waits for the device
+    // in hipFree) only now and then
+    const size_t step = (size_t)2 << 20;
+    size_t sz = std::max(bytes, e->arena_prev + e->arena_prev / 4);
+    sz = (sz + step - 1) / step * step;
     hipError_t err = hipMalloc(&e->arena, sz);
+    if (err != hipSuccess && sz > bytes) {
+        sz = bytes;  // the headroom does not fit: take what this build needs
+        err = hipMalloc(&e->arena, sz);
+    }
     if (err != hipSuccess) return hip_fail(err, "hipMalloc(arena)");
+    e->arena_prev = sz;
     e->arena_size = sz;
+    e->arena_pub = sz;
     return 0;
 }
 
@@ -197,6 +303,20 @@ static int ensure_fit(pripps_engine *e, size_t doubles) {
     if (err != hipSuccess) return hip_fail(err, "hipMalloc(fit scratch)");
     e->fit_doubles = sz;
     return 0;
+}
+
+// Fit scratch above this many doubles (64 MiB) is given back after the call
+// that needed it: a large grid fit should not keep that memory on the engine
+// (and out of a serving budget) for the engine's lifetime.  device_fit and
+// device_spline_total synchronise the stream before they return, so nothing
+// in flight still uses the buffer.  Caller holds e->mu.
+constexpr size_t kFitKeepDoubles = (size_t)8 << 20;
+
+static void trim_fit(pripps_engine *e) {
+    if (e->fit_doubles <= kFitKeepDoubles) return;
+    (void)hipFree(e->fit_buf);
+    e->fit_buf = nullptr;
+    e->fit_doubles = 0;
 }
 
 static void engine_release(pripps_engine *e) {
@@ -854,10 +974,13 @@ static int build_models(pripps_engine *e, int32_t ns, const pripps_debye_input *
 
     // models (persistent device memory)
     std::vector<pripps_debye_model *> models(ns, nullptr);
+    // a failed build's blocks go back to the pool only after the stream has
+    // drained (nothing in flight may still write them)
     auto cleanup = [&]() {
+        (void)hipStreamSynchronize(e->stream);
         for (auto *m : models) {
             if (m) {
-                if (m->dev) (void)hipFree(m->dev);
+                model_release_block(m);
                 delete m;
             }
         }
@@ -868,11 +991,10 @@ static int build_models(pripps_engine *e, int32_t ns, const pripps_debye_input *
         m->engine = e;
         m->nq = P[s].nq;
         m->q = P[s].q;
-        size_t bytes = std::max<size_t>(1, (size_t)11 * P[s].nq) * sizeof(double);
-        hipError_t err = hipMalloc((void **)&m->dev, bytes);
+        hipError_t err = model_alloc(m);
         if (err != hipSuccess) {
             cleanup();
-            return hip_fail(err, "hipMalloc(model)");
+            return hip_fail(err, "model buffer");
         }
     }
 
@@ -1039,6 +1161,10 @@ extern "C" int pripps_debye_model_fit(const pripps_debye_model *m, const double 
         return fail(PRIPPS_ERR_VALIDATION,
                     "grid fit requires at least 2 points per axis (got c1_points=" +
                         std::to_string(c1_points) + ", c2_points=" + std::to_string(c2_points) + ")");
+    // the device fit indexes grid points and data points with 32-bit ints
+    if (method == PRIPPS_FIT_GRID && (int64_t)c1_points * (int64_t)c2_points > INT32_MAX)
+        return fail(PRIPPS_ERR_VALIDATION, "grid fit: c1_points * c2_points exceeds 2^31 - 1");
+    if (n > INT32_MAX) return fail(PRIPPS_ERR_VALIDATION, "experimental data exceeds 2^31 - 1 points");
     const double *bb[2] = {c1b, c2b};
     const char *nm[2] = {"c1", "c2"};
     for (int k = 0; k < 2; ++k) {
@@ -1057,6 +1183,7 @@ extern "C" int pripps_debye_model_fit(const pripps_debye_model *m, const double 
     std::string err;
     rc = device_fit(e->stream, e->fit_buf, m->d_prof(), m->d_q(), m->nq, q, inten, sigma, n,
                     method, c1_points, c2_points, c1b, c2b, out, &err);
+    trim_fit(e);
     if (rc) return fail(rc, err);
     return PRIPPS_OK;
 }
@@ -1077,15 +1204,15 @@ extern "C" int pripps_debye_model_from_profiles(pripps_engine *e, const double *
     {
         std::lock_guard<std::mutex> lock(e->mu);
         hipError_t err = hipSetDevice(e->device);
-        if (err == hipSuccess)
-            err = hipMalloc((void **)&m->dev, std::max<size_t>(1, (size_t)11 * nq) * sizeof(double));
+        if (err == hipSuccess) err = model_alloc(m);
         if (err == hipSuccess && nq > 0)
             err = hipMemcpyAsync(m->d_prof(), profiles, (size_t)6 * nq * 8, hipMemcpyHostToDevice, e->stream);
         if (err == hipSuccess && nq > 0)
             err = hipMemcpyAsync(m->d_q(), q, (size_t)nq * 8, hipMemcpyHostToDevice, e->stream);
         if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
         if (err != hipSuccess) {
-            if (m->dev) (void)hipFree(m->dev);
+            (void)hipStreamSynchronize(e->stream);
+            model_release_block(m);
             delete m;
             return hip_fail(err, "pripps_debye_model_from_profiles");
         }
@@ -1101,13 +1228,15 @@ extern "C" int pripps_debye_model_stats(const pripps_debye_model *m, pripps_buil
     return PRIPPS_OK;
 }
 
+// The block returns to the pool without a device-wide wait.  Work that reads
+// it is on the engine's stream; a later model that receives the block writes
+// it only from a later submission on a stream whose work is ordered after
+// this call by the caller (every query of a model synchronises its stream
+// before returning), so the block is idle when it is handed out again.
 extern "C" int pripps_debye_model_free(pripps_debye_model *m) {
     if (!m) return PRIPPS_OK;
     pripps_engine *e = m->engine;
-    if (m->dev) {
-        (void)hipSetDevice(e->device);
-        (void)hipFree(m->dev);
-    }
+    model_release_block(m);
     delete m;
     engine_release(e);
     return PRIPPS_OK;
@@ -1129,6 +1258,7 @@ extern "C" int pripps_debye_model_fit_curve(const pripps_debye_model *m, double 
     std::string err;
     rc = device_spline_total(e->stream, e->fit_buf, m->d_prof(), m->d_q(), m->nq, vs, cd, q, n,
                              out, &err);
+    trim_fit(e);
     if (rc) return fail(rc, err);
     return PRIPPS_OK;
 }
@@ -1433,7 +1563,7 @@ extern "C" int pripps_debye_frames_average(pripps_engine *e, const pripps_debye_
         if (graph) (void)hipGraphDestroy(graph);
         if (work) (void)hipFree(work);
         if (model) {
-            if (model->dev) (void)hipFree(model->dev);
+            model_release_block(model);
             delete model;
         }
     };
@@ -1446,7 +1576,7 @@ extern "C" int pripps_debye_frames_average(pripps_engine *e, const pripps_debye_
         }                                    \
     } while (0)
     const size_t prof_n = (size_t)kProfiles * nq;
-    FRAMES_CHK(hipMalloc((void **)&model->dev, (size_t)11 * nq * sizeof(double)));
+    FRAMES_CHK(model_alloc(model));
     FRAMES_CHK(hipMalloc((void **)&work, ((size_t)C + 1) * prof_n * sizeof(double)));
     double *acc = work + (size_t)C * prof_n;
     FRAMES_CHK(launch_fill_double(acc, (int64_t)prof_n, 0.0, st));
@@ -1640,6 +1770,45 @@ int engine_build(pripps_engine *e, int32_t n, const pripps_debye_input *inputs, 
     return build_models(e, n, inputs, precision, tuning, out);
 }
 void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+// Device bytes one structure needs when built alone on `e`: the arena of its
+// plan (atoms, tables, work lists, item partials, chunk totals) and its model
+// block.  The plan is the one the build would use (same tuning table), so
+// the figure is exact for a build alone and a close upper bound per member
+// of a batch (only alignment padding is shared).  Returns false if the input
+// does not prepare (the build will report why).
+bool engine_estimate(pripps_engine *e, const pripps_debye_input *in, int32_t precision,
+                     uint64_t *arena_bytes, uint64_t *model_block) {
+    std::vector<Prepared> P(1);
+    const std::string saved = g_last_error;
+    const int rc = prepare(in, P[0]);
+    g_last_error = saved;
+    if (rc) return false;
+    BuildPlan B;
+    plan_build(e, P, precision, nullptr, 0, B);
+    *arena_bytes = B.total_bytes;
+    *model_block = (uint64_t)1 << pool_class(model_bytes(P[0].nq));
+    return true;
+}
+
+// Device bytes of e's arena (grow-only).
+uint64_t engine_arena_bytes(pripps_engine *e) { return e->arena_pub.load(); }
+
+// Free e's arena; the next build allocates a new one.  Called only while no
+// build runs on e.
+int engine_trim_arena(pripps_engine *e) {
+    std::lock_guard<std::mutex> lock(e->mu);
+    if (!e->arena) return PRIPPS_OK;
+    HIPCHK(hipSetDevice(e->device));
+    hipError_t err = hipFree(e->arena);
+    e->arena = nullptr;
+    e->arena_size = 0;
+    e->arena_pub = 0;
+    e->build_serial++;  // nothing of the freed arena is replayable
+    e->last_launches.clear();
+    if (err != hipSuccess) return hip_fail(err, "hipFree(arena)");
+    return PRIPPS_OK;
+}
 const std::string &get_last_error() { return g_last_error; }
 // used by the serving loop: a model built on a lane is handed to another
 // engine of the same device, so that its queries (intensity, profiles, fit)

@@ -38,6 +38,10 @@ int engine_build(pripps_engine *e, int32_t n, const pripps_debye_input *inputs, 
 void set_last_error(const std::string &msg);
 const std::string &get_last_error();
 void model_rehome(pripps_debye_model *m, pripps_engine *to);
+bool engine_estimate(pripps_engine *e, const pripps_debye_input *in, int32_t precision,
+                     uint64_t *arena_bytes, uint64_t *model_block);
+uint64_t engine_arena_bytes(pripps_engine *e);
+int engine_trim_arena(pripps_engine *e);
 void engine_retain(pripps_engine *e);
 void engine_unref(pripps_engine *e);
 }  // namespace pripps
@@ -54,7 +58,8 @@ struct Request {
     std::vector<double> x, y, z, q, ffa, ffe, ffw, w;
     std::vector<int32_t> kind;
     pripps_debye_input in{};
-    uint64_t bytes = 0;
+    uint64_t arena = 0;  // device bytes of its plan's arena when built alone
+    uint64_t model_bytes = 0;  // device bytes of its model block
     // completion
     bool done = false;
     bool waiting = false;  // a pripps_server_wait call holds this request
@@ -71,15 +76,10 @@ struct Request {
 // gain from a fourth).
 constexpr int kLanes = 3;
 
-uint64_t estimate_bytes(const pripps_debye_input &in) {
-    // model (11 nq doubles) + sorted atoms + tables + a share of the partial
-    // profile buffer (one 6 x 64 FP64 slot per 64 x 256 pair tile and strip)
-    const double n = (double)in.n_atoms, nq = (double)in.nq;
-    const double tiles = n / 64.0 + 16.0;
-    const double spans = n / 256.0 + 1.0;
-    const double strips = nq / 64.0 + 1.0;
-    const double partial = tiles * spans * strips * 6.0 * 64.0 * 8.0 * 0.5;
-    return (uint64_t)(11.0 * nq * 8.0 + n * 40.0 + 2.0 * 16.0 * nq * 8.0 + partial + 4096.0);
+// Fallback footprint of an input that does not prepare (its build fails at
+// validation and needs no arena): the model block.
+uint64_t fallback_bytes(const pripps_debye_input &in) {
+    return (uint64_t)(11.0 * (double)std::max<int32_t>(in.nq, 1) * 8.0) + 4096;
 }
 
 }  // namespace
@@ -95,48 +95,83 @@ struct pripps_server {
     std::deque<Request *> queue;
     std::unordered_map<int64_t, Request *> all;
     int64_t next_id = 1;
-    uint64_t in_use = 0;  // estimated bytes of batches on the device
+    // Admission.  `budget` is fixed when the server is created: the
+    // configured figure, or 90 % of the device memory free at that moment.
+    // Charged against it: every lane's arena (grow-only, it stays allocated
+    // between batches) or, while the lane builds, the larger of that arena
+    // and the arena its batch needs, plus the batch's model blocks until they
+    // are handed over.  Models the caller holds after pripps_server_wait are
+    // the caller's memory, outside the budget.  A request whose own arena and
+    // model exceed the budget can never run and fails at once with
+    // PRIPPS_ERR_OUT_OF_MEMORY; any other waits until it fits, and idle lanes
+    // give their arenas back if that is what keeps it out.
+    uint64_t budget = 0;
+    std::vector<uint64_t> lane_need;    // arena bytes the lane's current batch needs (0: idle)
+    std::vector<uint64_t> lane_models;  // model bytes of the lane's current batch
     bool stopping = false;
 
-    void worker(pripps_engine *lane) {
+    // device bytes charged to lane k (caller holds mu)
+    uint64_t charged(int k) const {
+        return std::max(engine_arena_bytes(lanes[k]), lane_need[k]) + lane_models[k];
+    }
+
+    void worker(int me) {
+        pripps_engine *lane = lanes[me];
         for (;;) {
             std::vector<Request *> batch;
-            uint64_t batch_bytes = 0;
             {
                 std::unique_lock<std::mutex> lk(mu);
                 cv_work.wait(lk, [&] { return stopping || !queue.empty(); });
                 if (stopping && queue.empty()) return;
-                uint64_t free_b = 0, total_b = 0;
-                lk.unlock();
-                pripps_engine_memory(lane, &free_b, &total_b);
-                lk.lock();
-                uint64_t budget = cfg.memory_budget ? cfg.memory_budget : (uint64_t)(0.9 * (double)free_b);
                 const int cap = cfg.max_in_flight > 0 ? cfg.max_in_flight : 256;
+                uint64_t others = 0, idle_held = 0;
+                for (int k = 0; k < (int)lanes.size(); ++k) {
+                    if (k == me) continue;
+                    others += charged(k);
+                    if (lane_need[k] == 0) idle_held += engine_arena_bytes(lanes[k]);
+                }
+                const uint64_t own = engine_arena_bytes(lane);
+                idle_held += own;
+                uint64_t need = 0, models = 0;
+                bool head_blocked = false;
                 while (!queue.empty() && (int)batch.size() < cap) {
                     Request *r = queue.front();
-                    if (r->bytes > budget) {
+                    if (r->arena + r->model_bytes > budget) {
                         // can never fit: fail it rather than stall the queue
                         queue.pop_front();
                         r->done = true;
                         r->status = PRIPPS_ERR_OUT_OF_MEMORY;
-                        r->error = "request needs about " + std::to_string(r->bytes) +
+                        r->error = "request needs about " + std::to_string(r->arena + r->model_bytes) +
                                    " bytes of device memory; the serving budget is " +
                                    std::to_string(budget);
                         r->latency_ms = std::chrono::duration<double, std::milli>(Clock::now() - r->t_submit).count();
                         cv_done.notify_all();
                         continue;
                     }
-                    if (in_use + batch_bytes + r->bytes > budget) break;  // wait for memory
+                    const uint64_t mine = std::max(own, need + r->arena) + models + r->model_bytes;
+                    if (others + mine > budget) {  // wait for memory
+                        head_blocked = batch.empty();
+                        break;
+                    }
                     queue.pop_front();
                     batch.push_back(r);
-                    batch_bytes += r->bytes;
+                    need += r->arena;
+                    models += r->model_bytes;
                 }
                 if (batch.empty()) {
+                    if (head_blocked && idle_held > 0) {
+                        // idle lanes' arenas (this one's included) keep the
+                        // head out: give them back
+                        for (int k = 0; k < (int)lanes.size(); ++k)
+                            if (k == me || lane_need[k] == 0) (void)engine_trim_arena(lanes[k]);
+                        continue;
+                    }
                     // the head does not fit while other batches hold memory
                     cv_work.wait_for(lk, std::chrono::milliseconds(1));
                     continue;
                 }
-                in_use += batch_bytes;
+                lane_need[me] = std::max<uint64_t>(need, 1);
+                lane_models[me] = models;
             }
             const size_t nb = batch.size();
             std::vector<pripps_debye_input> ins(nb);
@@ -162,7 +197,8 @@ struct pripps_server {
             const auto now = Clock::now();
             {
                 std::lock_guard<std::mutex> lk(mu);
-                in_use -= batch_bytes;
+                lane_need[me] = 0;
+                lane_models[me] = 0;
                 for (size_t k = 0; k < nb; ++k) {
                     Request *r = batch[k];
                     r->done = true;
@@ -196,6 +232,7 @@ extern "C" int pripps_server_create(pripps_engine *engine, const pripps_server_c
     if (rc) return rc;
     auto *s = new pripps_server();
     if (config) s->cfg = *config;
+    s->budget = s->cfg.memory_budget ? s->cfg.memory_budget : (uint64_t)(0.9 * (double)f);
     if (s->cfg.precision != PRIPPS_FP32_COMPENSATED && s->cfg.precision != PRIPPS_FP64) {
         delete s;
         set_last_error("pripps_server_create: precision must be FP32_COMPENSATED or FP64");
@@ -217,7 +254,9 @@ extern "C" int pripps_server_create(pripps_engine *engine, const pripps_server_c
     }
     engine_retain(engine);  // released by pripps_server_destroy
     s->base = engine;
-    for (auto *lane : s->lanes) s->workers.emplace_back([s, lane] { s->worker(lane); });
+    s->lane_need.assign(kLanes, 0);
+    s->lane_models.assign(kLanes, 0);
+    for (int k = 0; k < kLanes; ++k) s->workers.emplace_back([s, k] { s->worker(k); });
     *out = s;
     return PRIPPS_OK;
 }
@@ -258,7 +297,11 @@ extern "C" int pripps_server_submit(pripps_server *s, const pripps_debye_input *
     r->in.excluded_ff = in->excluded_ff ? r->ffe.data() : nullptr;
     r->in.water_ff = in->water_ff ? r->ffw.data() : nullptr;
     r->in.hydration_weight = in->water_ff ? r->w.data() : nullptr;
-    r->bytes = estimate_bytes(*in);
+    // footprint from the plan the build will use (outside the server lock)
+    if (!engine_estimate(s->lanes[0], &r->in, s->cfg.precision, &r->arena, &r->model_bytes)) {
+        r->arena = 0;
+        r->model_bytes = fallback_bytes(*in);
+    }
     {
         std::lock_guard<std::mutex> lk(s->mu);
         r->id = s->next_id++;
@@ -299,7 +342,9 @@ extern "C" int pripps_server_wait(pripps_server *s, int64_t id, int32_t timeout_
         set_last_error("request " + std::to_string(id) + " still running");
         return PRIPPS_ERR_NOT_READY;
     }
-    s->all.erase(it);
+    // by key: submits during the wait may have rehashed the map, which
+    // invalidates the iterator found above
+    s->all.erase(id);
     const int status = r->status;
     const std::string err = r->error;
     *model = r->model;

@@ -161,3 +161,61 @@ def test_server_bad_request_fails_alone(engine, foxs_engine):
             srv.wait(slow)
     for k, m in got.items():
         assert np.array_equal(m.profiles(), engine.build(inps[k], "fp32").profiles())
+
+
+def test_served_stream_c5(engine, oracle, foxs_engine):
+    """The served configuration at its own sizes: 40 requests of the request
+    stream (N log-U[300, 30000], the largest at 29,013 atoms; nq U[200, 800];
+    q on [0, 0.5]; FoXS solvent with surface hydration weights), offered as a
+    Poisson stream at 2,000 requests/s -- above the loop's saturation, so
+    requests queue and batch.  Each is followed by five intensity() calls at
+    random knobs.  A sample of results equals the same structure built alone
+    on a fresh engine, bit for bit, and the largest request is within 1e-5 of
+    the oracle at three q."""
+    import time
+    from pripps_amd.engine import Engine
+    from pripps_amd.server import Server
+    from pripps_amd.synth import ELEMENTS, globule, linear_q, radius_for, request_stream
+    ns, nqs, knobs, seeds = request_stream(40, seed=21)
+    assert ns.max() >= 25000
+    lut = foxs_engine.names_to_ids(list(ELEMENTS))
+    inps = []
+    for n, nq, sd in zip(ns, nqs, seeds):
+        names, pos, w = globule(int(n), radius_for(int(n)), int(sd), solvent=True)
+        idx = np.array([ELEMENTS.index(e) for e in names])
+        inps.append(foxs_engine.prepare_input(pos, lut[idx], linear_q(0.0, 0.5, int(nq)),
+                                              excluded="foxs", hydration="weights",
+                                              hydration_weights=w))
+    print("c5: 40 requests prepared", flush=True)
+    arrivals = np.cumsum(np.random.default_rng(3).exponential(1.0 / 2000.0, size=len(inps)))
+    totals = {}
+    with Server(engine, "fp32") as srv:
+        t0 = time.perf_counter()
+        ids = []
+        for k, inp in enumerate(inps):
+            delay = t0 + arrivals[k] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            ids.append(srv.submit(inp))
+        models = []
+        for k, rid in enumerate(ids):
+            m, lat = srv.wait(rid)
+            assert lat > 0
+            for vs, cd in knobs[k]:
+                tot = m.intensity(vs, cd)[0]
+                assert np.all(np.isfinite(tot))
+            totals[k] = tot
+            models.append(m)
+        wall = time.perf_counter() - t0
+        print(f"c5: 40 requests served in {wall:.2f} s", flush=True)
+        profs = [m.profiles() for m in models]
+    fresh = Engine(0)
+    big = int(np.argmax(ns))
+    sample = sorted({big, *np.linspace(0, len(inps) - 1, 8).round().astype(int).tolist()})
+    for k in sample:
+        alone = fresh.build(inps[k], "fp32")
+        assert np.array_equal(profs[k], alone.profiles()), k
+        assert np.array_equal(totals[k], alone.intensity(*knobs[k][-1])[0]), k
+    sel = np.array([1, int(nqs[big]) // 2, int(nqs[big]) - 1])
+    ref = oracle_at(oracle, inps[big], sel)
+    assert envelope_error(profs[big][:, sel], ref).max() <= 1e-5

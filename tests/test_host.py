@@ -273,3 +273,75 @@ def test_write_intensity_csv(tmp_path):
     bare = pripps_amd.Intensity(q[:1], np.array([12.5]))
     pripps_amd.write_intensity_csv(path, bare)
     assert path.read_text() == "q,total\n0.000000,12.500000\n"
+
+
+def test_linear_q_grid_is_the_oracle_grid_bitwise(oracle):
+    """The product's q grid (Pripps.debye_model's qmin/qmax/nq) against the
+    oracle's restatement of src/q_sampler.rs:24-29, bit for bit, and the
+    reference's endpoint tests (src/q_sampler.rs:173-185)."""
+    from pripps_amd.synth import linear_q
+    for qmin, qmax, nq in ((0.0, 0.5, 500), (0.0, 0.3, 20), (0.0, 0.2, 8), (0.005, 1.0, 1000),
+                           (0.013, 0.7391, 777), (0.42, 0.42, 1), (0.1, 0.2, 2), (0.0, 1.0, 0)):
+        want = oracle.linear_q_values(qmin, qmax, nq)
+        got = pripps_amd.linear_q_values(qmin, qmax, nq)
+        assert got.dtype == np.float64 and got.tobytes() == want.tobytes(), (qmin, qmax, nq)
+        assert linear_q(qmin, qmax, nq).tobytes() == want.tobytes()
+    q = pripps_amd.linear_q_values(0.0, 1.0, 11)
+    assert len(q) == 11 and q[0] == 0.0 and q[10] == 1.0
+    assert pripps_amd.linear_q_values(0.42, 0.42, 1).tolist() == [0.42]
+    with pytest.raises(E.ValidationError):
+        pripps_amd.linear_q_values(0.0, 0.5, -1)
+
+
+def _yaml_tagged(path):
+    """The form-factor YAML read with a plain safe loader that keeps each
+    tagged node as {"tag": ..., **mapping} -- independent of the package's
+    own resolver."""
+    import yaml
+
+    class L(yaml.SafeLoader):
+        pass
+
+    def tagged(loader, suffix, node):
+        return {"tag": suffix, **loader.construct_mapping(node, deep=True)}
+
+    L.add_multi_constructor("!", tagged)
+    with open(path) as fh:
+        return yaml.load(fh, Loader=L)
+
+
+def test_per_kind_excluded_table(oracle):
+    """ExcludedVolume::PerKind (src/solvent/per_kind_excluded.rs:16-39): each
+    kind's `excluded_solvent` entry of the same YAML, evaluated at q (the
+    polynomial of src/formfactor.rs:379-385 through the oracle)."""
+    from pripps_amd.solvent import excluded_table
+    path = os.path.join(REFDATA, "poly_amino_acid.yaml")
+    ff = read_formfactors(path)
+    raw = _yaml_tagged(path)
+    st = pripps_amd.structure_from_file(os.path.join(REFDATA, "4lzt.xyz"), ff.kinds())
+    used = sorted(set(st.ids.tolist()))
+    q = pripps_amd.linear_q_values(0.0, 0.5, 51)
+    tab = excluded_table("per-kind", ff, used, q, 0.334)
+    assert tab.shape == (len(used), len(q)) and len(used) == 20
+    for r, kid in enumerate(used):
+        spec = raw[ff.entries[kid].kind.name]["excluded_solvent"]
+        assert spec["tag"] == "Polynomial"
+        want = [oracle.eval_polynomial(spec["coeff"], x) for x in q]
+        assert np.array_equal(tab[r], want), ff.entries[kid].kind.name
+
+
+def test_per_kind_excluded_reference_cases(tmp_path):
+    """src/solvent/per_kind_excluded.rs:58-90: a kind whose excluded_solvent is
+    the constant 3.5 gives 3.5; a kind without one is a validation error
+    naming `excluded_solvent`."""
+    from pripps_amd.solvent import excluded_table
+    (tmp_path / "a.yaml").write_text(
+        "ALA: { formfactor: !Polynomial { qrange: [0.0, 1.0], coeff: [10.0] },\n"
+        "       excluded_solvent: !Polynomial { qrange: [0.0, 1.0], coeff: [3.5] } }\n")
+    (tmp_path / "b.yaml").write_text(
+        "ALA: { formfactor: !Polynomial { qrange: [0.0, 1.0], coeff: [10.0] } }\n")
+    ff = read_formfactors(str(tmp_path / "a.yaml"))
+    assert excluded_table("per-kind", ff, [0], np.array([0.2]), 0.334).tolist() == [[3.5]]
+    ff = read_formfactors(str(tmp_path / "b.yaml"))
+    with pytest.raises(E.ValidationError, match="excluded_solvent"):
+        excluded_table("per-kind", ff, [0], np.array([0.2]), 0.334)
