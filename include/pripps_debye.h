@@ -210,6 +210,30 @@ int pripps_debye_model_recompute(pripps_debye_model *model, int32_t repeats,
                                  double *pair_ms_mean, double *total_ms);
 int pripps_debye_model_free(pripps_debye_model *model);
 
+/*
+ * Build without device allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Computes
+ * the six profiles of one structure like pripps_debye_model_build with the
+ * same plan (same tuning, or the default plan), but every device buffer it
+ * needs beyond the caller's arrays is one fixed device array of
+ * PRIPPS_NOALLOC_SCRATCH_BYTES per GPU, held by the library until the
+ * process ends; the call makes no hipMalloc, hipFree or other allocating
+ * device call.  `profiles_out` (host, [6][nq], DebyeTerms order) receives
+ * the profiles; no model is created.  Without an explicit tuning span, a
+ * default plan whose item partials do not fit is built with longer j-spans
+ * (fewer items) instead, which changes the FP32 summation order but not the
+ * precision bound; a structure that does not fit even so is refused with
+ * PRIPPS_ERR_OUT_OF_MEMORY (FP32 on a uniform grid of 500 q: about 65,000
+ * atoms; FP64 or longer grids: fewer).  Calls
+ * on one GPU share the array and are serialised by the library.  Precision:
+ * FP32_COMPENSATED or FP64.
+ */
+#define PRIPPS_NOALLOC_SCRATCH_BYTES (64u << 20)
+int pripps_debye_build_noalloc(pripps_engine *engine, const pripps_debye_input *input,
+                               int32_t precision, const pripps_tuning *tuning,
+                               double *profiles_out);
+
 /* ---------------------------------------------------------------------- */
 /* Autotuning.  No reference counterpart (the reference has no kernels).   */
 /* ---------------------------------------------------------------------- */

@@ -27,6 +27,9 @@ PRECISIONS = {"fp32": FP32_COMPENSATED, "fp32_compensated": FP32_COMPENSATED,
 FIT_NELDER_MEAD = 0
 FIT_GRID = 1
 
+# bytes of the fixed device scratch of pripps_debye_build_noalloc (per GPU)
+NOALLOC_SCRATCH_BYTES = 64 << 20
+
 _D = ctypes.POINTER(ctypes.c_double)
 _I32 = ctypes.POINTER(ctypes.c_int32)
 
@@ -92,7 +95,7 @@ EXPORTS = (
     "pripps_debye_model_profiles", "pripps_debye_model_fit", "pripps_debye_model_fit_curve",
     "pripps_debye_model_from_profiles",
     "pripps_debye_model_stats", "pripps_debye_model_recompute",
-    "pripps_debye_model_free", "pripps_engine_autotune", "pripps_engine_tuning_table",
+    "pripps_debye_model_free", "pripps_debye_build_noalloc", "pripps_engine_autotune", "pripps_engine_tuning_table",
     "pripps_engine_set_tuning_entry", "pripps_debye_frames_average", "pripps_server_create",
     "pripps_server_submit", "pripps_server_wait", "pripps_server_destroy",
     "pripps_direct_model_build", "pripps_direct_model_n_groups", "pripps_direct_model_q_values",
@@ -144,6 +147,8 @@ def load():
                                                    ctypes.POINTER(ctypes.c_double)]
         L.pripps_debye_model_stats.argtypes = [vp, ctypes.POINTER(BuildStats)]
         L.pripps_debye_model_free.argtypes = [vp]
+        L.pripps_debye_build_noalloc.argtypes = [vp, ctypes.POINTER(DebyeInput), ctypes.c_int32,
+                                                 ctypes.POINTER(Tuning), _D]
         L.pripps_engine_autotune.argtypes = [vp, ctypes.POINTER(DebyeInput), ctypes.c_int32,
                                              ctypes.c_int32, ctypes.POINTER(Tuning),
                                              ctypes.POINTER(ctypes.c_double)]

@@ -1098,6 +1098,100 @@ extern "C" int pripps_debye_model_build_batch(pripps_engine *e, int32_t n,
 }
 
 // ---------------------------------------------------------------------------
+// build without device allocation
+// ---------------------------------------------------------------------------
+// Every device buffer of pripps_debye_build_noalloc lives in this array:
+// atoms, tables, structure record, work lists, queue head, item partials,
+// chunk totals, q grid and the output profiles.  Each GPU the library runs
+// on holds its own copy for the life of the process.  In host code the name
+// `scratch` is only a host-side stand-in; kernels receive the device address
+// that hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) char scratch[PRIPPS_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+struct NoAllocDevice {
+    std::mutex mu;          // one call at a time per GPU (the array is shared)
+    char *ptr = nullptr;    // device address of `scratch` on this GPU
+};
+
+NoAllocDevice &noalloc_device(int device) {
+    static std::mutex mu;
+    static std::vector<std::unique_ptr<NoAllocDevice>> devs;
+    std::lock_guard<std::mutex> lk(mu);
+    if ((int)devs.size() <= device) devs.resize((size_t)device + 1);
+    if (!devs[device]) devs[device].reset(new NoAllocDevice());
+    return *devs[device];
+}
+}  // namespace
+
+extern "C" int pripps_debye_build_noalloc(pripps_engine *e, const pripps_debye_input *input,
+                                          int32_t precision, const pripps_tuning *tune,
+                                          double *profiles_out) {
+    if (!e || !input) return fail(PRIPPS_ERR_USAGE, "engine and input must be non-NULL");
+    if (precision != PRIPPS_FP32_COMPENSATED && precision != PRIPPS_FP64)
+        return fail(PRIPPS_ERR_USAGE, "noalloc build: precision must be FP32_COMPENSATED or FP64");
+    std::vector<Prepared> P(1);
+    int rc = prepare(input, P[0]);
+    if (rc) return rc;
+    const int32_t nq = P[0].nq;
+    if (nq > 0 && !profiles_out) return fail(PRIPPS_ERR_USAGE, "profiles_out is NULL");
+    if (nq == 0) return PRIPPS_OK;
+    // the q grid and the profiles sit in the caller region of the upload
+    const size_t user = (size_t)7 * nq * sizeof(double);
+    BuildPlan B;
+    plan_build(e, P, precision, tune, user, B);
+    // Without an explicit span, a plan whose partials do not fit takes longer
+    // j-spans (fewer items, so fewer partial slots) until it fits.
+    const bool span_given = tune && tune->span > 0;
+    while (B.total_bytes > PRIPPS_NOALLOC_SCRATCH_BYTES && !span_given && B.sp[0].span < 16384) {
+        pripps_tuning t{};
+        t.q_per_lane = B.sp[0].Q;
+        t.block = B.sp[0].block;
+        t.span = std::min(16384, 2 * B.sp[0].span);
+        t.variant = B.sp[0].variant;
+        BuildPlan wider;
+        plan_build(e, P, precision, &t, user, wider);
+        B = std::move(wider);
+    }
+    if (B.total_bytes > PRIPPS_NOALLOC_SCRATCH_BYTES)
+        return fail(PRIPPS_ERR_OUT_OF_MEMORY,
+                    "noalloc build: this structure needs " + std::to_string(B.total_bytes) +
+                        " bytes of device scratch; the fixed scratch holds " +
+                        std::to_string((size_t)PRIPPS_NOALLOC_SCRATCH_BYTES) +
+                        " (use pripps_debye_model_build)");
+    // host staging of the upload (pageable: no pinned allocation)
+    std::vector<char> host(B.upload_bytes);
+
+    NoAllocDevice &D = noalloc_device(e->device);
+    std::lock_guard<std::mutex> dlock(D.mu);
+    std::lock_guard<std::mutex> lock(e->mu);
+    HIPCHK(hipSetDevice(e->device));
+    if (!D.ptr) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(PRIPPS_ERR_DEVICE, "noalloc build: no device address for the scratch array");
+        D.ptr = (char *)p;
+    }
+    char *dbase = D.ptr;
+    double *d_q = (double *)(dbase + B.o_user);
+    double *d_prof = d_q + nq;
+    std::vector<const double *> dq(1, d_q);
+    std::vector<double *> dprof(1, d_prof);
+    stage_build(B, P, host.data(), dbase, dq, dprof);
+    std::memcpy(host.data() + B.o_user, P[0].q.data(), (size_t)nq * sizeof(double));
+    std::vector<LaunchRecord> launches;
+    rc = make_launches(e, B, dbase, launches);
+    if (rc) return rc;
+    hipStream_t st = e->stream;
+    HIPCHK(hipMemcpyAsync(dbase, host.data(), B.upload_bytes, hipMemcpyHostToDevice, st));
+    HIPCHK(enqueue_kernels(B, P, dbase, launches, st));
+    HIPCHK(hipMemcpyAsync(profiles_out, d_prof, (size_t)kProfiles * nq * sizeof(double),
+                          hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return PRIPPS_OK;
+}
+
+// ---------------------------------------------------------------------------
 // model queries
 // ---------------------------------------------------------------------------
 extern "C" int pripps_debye_model_nq(const pripps_debye_model *m, int32_t *nq) {
