@@ -147,7 +147,7 @@ namespace pripps {
 // the inputs still resident in the engine arena.
 struct LaunchRecord {
     bool fp64 = false, uniform = true;
-    int Q = 64, blocks = 0, n_strips = 0;
+    int Q = 64, P = 0, blocks = 0, n_strips = 0;
     int64_t n_items = 0;
     int *counter = nullptr;
     TiledArgs targs{};
@@ -159,7 +159,8 @@ hipError_t run_launch(const LaunchRecord &r, hipStream_t st, hipEvent_t ev_a, hi
     hipError_t err = launch_fill_int(r.counter, 1, 0, st);
     if (err == hipSuccess && ev_a) err = hipEventRecord(ev_a, st);
     if (err == hipSuccess && r.n_items > 0)
-        err = launch_tiled(r.fp64, r.Q, r.uniform, r.targs, r.blocks, st);
+        err = r.Q == kChebQ ? launch_cheb(r.P, r.targs, r.blocks, st)
+                            : launch_tiled(r.fp64, r.Q, r.uniform, r.targs, r.blocks, st);
     if (err == hipSuccess && ev_b) err = hipEventRecord(ev_b, st);
     if (err == hipSuccess && r.n_strips > 0) err = launch_finalize(r.fargs, r.n_strips, st);
     return err;
@@ -369,6 +370,7 @@ struct Prepared {
     int8_t cls_wet[kMaxClasses]{};
     std::vector<int4> tiles;  // (i0, count, kind, 0)
     double pair_q_terms = 0;
+    double rmax = 0;          // upper bound of every pair distance (A)
 };
 
 bool finite_all(const double *p, size_t n) {
@@ -519,6 +521,7 @@ int prepare(const pripps_debye_input *in, Prepared &P) {
         rmax2 = std::max(rmax2, p4[0] * p4[0] + p4[1] * p4[1] + p4[2] * p4[2]);
     }
     const double rmax = 2.0 * std::sqrt(rmax2) * (1.0 + 1e-6) + 1e-6;
+    P.rmax = rmax;
     P.small_h = std::fabs(P.dq) * rmax / (4.0 * M_PI) <= 0.125;
     double qabs = 0;
     for (double v : P.q) qabs = std::max(qabs, std::fabs(v));
@@ -551,11 +554,73 @@ namespace {
 
 struct StructPlan {
     int Q = 64, span = 256, block = 64, variant = 3;
+    // Chebyshev plans (Q == kChebQ): expansion order and strips
+    int P = 0;
+    std::vector<int32_t> strip_first;  // [n_strips + 1]
+    std::vector<double> strip_geo;     // [n_strips][2]: centre, half-width
+    int n_strips(int nq) const {
+        return Q == kChebQ ? (int)strip_first.size() - 1 : (nq + Q - 1) / Q;
+    }
 };
 
+// Q: q-points per lane strip of the recurrence kernels (32, 64, 128), or
+// kChebQ for the Chebyshev kernel (FP32, any grid).
 int valid_q(int precision, bool uniform, int Q) {
     if (precision == PRIPPS_FP64) return (Q == 32 || (Q == 64 && uniform)) ? Q : 0;
-    return (Q == 32 || Q == 64 || (Q == 128 && uniform)) ? Q : 0;
+    return (Q == 32 || Q == 64 || (Q == 128 && uniform) || Q == kChebQ) ? Q : 0;
+}
+
+// Strips of a Chebyshev plan: runs of up to kChebQ consecutive q-points
+// whose half-width H keeps beta = H r_max within what order P admits.  Each
+// built order is tried with greedy strips and the one with the fewest
+// instructions per pair over the grid wins (about 60 + 4.5 P per pair and
+// strip).  The expansion's error is about uniform in absolute terms across a
+// strip, while the profiles divide the pair sum by q, so a strip also keeps
+// H <= kChebLowQ * (its smallest non-zero q): that bounds the amplification
+// of the error at the strip's low end (q = 0 itself takes the closed form).
+constexpr double kChebLowQ = 128.0;
+
+void cheb_strips(const Prepared &p, StructPlan &sp) {
+    static const int orders[] = {12, 16, 20, 24, 28, 32, 40};
+    static double limit[7] = {0};
+    static std::once_flag once;
+    std::call_once(once, [] {
+        for (int k = 0; k < 7; ++k) limit[k] = cheb_beta_limit(orders[k]);
+    });
+    const int nq = p.nq;
+    double best_cost = 0;
+    for (int k = 0; k < 7; ++k) {
+        const double hmax = limit[k] / std::max(p.rmax, 1e-300);
+        std::vector<int32_t> first{0};
+        std::vector<double> geo;
+        int m0 = 0;
+        while (m0 < nq) {
+            double lo = p.q[m0], hi = p.q[m0];
+            double nz = p.q[m0] > 0.0 ? p.q[m0] : HUGE_VAL;  // smallest non-zero q
+            int m1 = m0 + 1;
+            while (m1 < nq && m1 - m0 < kChebQ) {
+                const double l2 = std::min(lo, p.q[m1]), h2 = std::max(hi, p.q[m1]);
+                const double nz2 = p.q[m1] > 0.0 ? std::min(nz, p.q[m1]) : nz;
+                const double half = 0.5 * (h2 - l2);
+                if (half > hmax || (nz2 < HUGE_VAL && half > kChebLowQ * nz2)) break;
+                lo = l2;
+                hi = h2;
+                nz = nz2;
+                ++m1;
+            }
+            first.push_back(m1);
+            geo.push_back(0.5 * (lo + hi));
+            geo.push_back(0.5 * (hi - lo));
+            m0 = m1;
+        }
+        const double cost = (double)(first.size() - 1) * (60.0 + 4.5 * orders[k]);
+        if (k == 0 || cost < best_cost) {
+            best_cost = cost;
+            sp.P = orders[k];
+            sp.strip_first = std::move(first);
+            sp.strip_geo = std::move(geo);
+        }
+    }
 }
 
 // Uniform grids: the longest strip whose accumulators fit, so that one
@@ -578,8 +643,8 @@ int default_q(int precision, bool uniform, int nq, int n_atoms) {
 // structures on).  It depends on the structure only, so a structure is cut
 // into the same items alone or in any batch, and its profiles are bitwise
 // the same either way.
-int default_span(const Prepared &p, int Q, int n_cu) {
-    const double strips = (double)((p.nq + Q - 1) / Q);
+int default_span(const Prepared &p, int Q, int n_cu, int n_strips = 0) {
+    const double strips = n_strips > 0 ? (double)n_strips : (double)((p.nq + Q - 1) / Q);
     const double work = (double)p.tiles.size() * 0.5 * p.n * strips;
     const double slots = std::max(1, n_cu) * 8.0;
     const int span = (int)(work / (16.0 * slots));
@@ -619,7 +684,7 @@ int log2_nq_of(int nq) {
 // persistent launch.
 struct LaunchGroup {
     bool uniform = true;
-    int Q = 64, variant = 3;
+    int Q = 64, variant = 3, P = 0;
     std::vector<int> members;
     std::vector<Item> items;
     std::vector<int32_t> csr_ptr, csr_items, strip_struct, strip_index, stage_ptr;
@@ -635,7 +700,7 @@ struct BuildPlan {
     int ns = 0;
     std::vector<StructPlan> sp;
     std::vector<LaunchGroup> groups;
-    std::vector<size_t> o_atoms, o_ffa, o_ffe, o_ffw, o_kn, o_kw, o_tiles, o_pp;
+    std::vector<size_t> o_atoms, o_ffa, o_ffe, o_ffw, o_kn, o_kw, o_tiles, o_pp, o_sf, o_sg;
     size_t o_structs = 0, o_user = 0, upload_bytes = 0, total_bytes = 0;
     int64_t n_items = 0;
 };
@@ -663,7 +728,8 @@ static StructPlan resolve_plan(pripps_engine *e, const pripps_tuning *tune, int 
                  : have ? valid_q(precision, p.uniform, te.q_per_lane) : 0;
     if (!Q) Q = default_q(precision, p.uniform, p.nq, p.n);
     sp.Q = Q;
-    const int base = default_span(p, Q, e->n_cu);
+    if (Q == kChebQ) cheb_strips(p, sp);
+    const int base = default_span(p, Q, e->n_cu, Q == kChebQ ? sp.n_strips(p.nq) : 0);
     if (expl && tune->span > 0) sp.span = std::max(64, tune->span);
     else if (have) sp.span = shifted_span(base, te.span_shift);
     else sp.span = base;
@@ -684,7 +750,7 @@ static void plan_group(LaunchGroup &g, const std::vector<Prepared> &P,
     auto each_item = [&](auto &&emit) {
         for (int s : g.members) {
             const Prepared &p = P[s];
-            const int nst = (p.nq + g.Q - 1) / g.Q;
+            const int nst = sp[s].n_strips(p.nq);
             const int span = sp[s].span;
             for (const int4 &t : p.tiles)
                 for (int32_t j0 = t.x; j0 < p.n; j0 += span)
@@ -710,7 +776,7 @@ static void plan_group(LaunchGroup &g, const std::vector<Prepared> &P,
     std::vector<int> local(P.size(), -1);
     for (int k = 0; k < nm; ++k) {
         local[g.members[k]] = k;
-        base[k + 1] = base[k] + (P[g.members[k]].nq + g.Q - 1) / g.Q;
+        base[k + 1] = base[k] + sp[g.members[k]].n_strips(P[g.members[k]].nq);
     }
     const int64_t n_strips = base[nm];
     std::vector<int32_t> cnt(n_strips + 1, 0);
@@ -749,7 +815,8 @@ static void plan_build(pripps_engine *e, const std::vector<Prepared> &P, int pre
             B.sp[s] = resolve_plan(e, tune, precision, P[s]);
             LaunchGroup *g = nullptr;
             for (auto &x : B.groups)
-                if (x.Q == B.sp[s].Q && x.uniform == P[s].uniform && x.variant == B.sp[s].variant)
+                if (x.Q == B.sp[s].Q && x.uniform == P[s].uniform && x.variant == B.sp[s].variant &&
+                    x.P == B.sp[s].P)
                     g = &x;
             if (!g) {
                 B.groups.emplace_back();
@@ -757,6 +824,7 @@ static void plan_build(pripps_engine *e, const std::vector<Prepared> &P, int pre
                 g->Q = B.sp[s].Q;
                 g->uniform = P[s].uniform;
                 g->variant = B.sp[s].variant;
+                g->P = B.sp[s].P;
             }
             g->members.push_back(s);
         }
@@ -767,7 +835,8 @@ static void plan_build(pripps_engine *e, const std::vector<Prepared> &P, int pre
     }
     Layout L;
     const size_t atom_bytes = B.fp64 ? sizeof(Atom<double>) : sizeof(Atom<float>);
-    for (auto *v : {&B.o_atoms, &B.o_ffa, &B.o_ffe, &B.o_ffw, &B.o_kn, &B.o_kw, &B.o_tiles, &B.o_pp})
+    for (auto *v : {&B.o_atoms, &B.o_ffa, &B.o_ffe, &B.o_ffw, &B.o_kn, &B.o_kw, &B.o_tiles, &B.o_pp,
+                    &B.o_sf, &B.o_sg})
         v->assign(ns, 0);
     for (int s = 0; s < ns; ++s) {
         B.o_atoms[s] = L.take(std::max<size_t>(1, (size_t)P[s].n) * atom_bytes);
@@ -776,6 +845,10 @@ static void plan_build(pripps_engine *e, const std::vector<Prepared> &P, int pre
         B.o_ffw[s] = L.take(P[s].ffw.size() * 8 + 8);
         B.o_kn[s] = L.take(P[s].K * 8);
         B.o_kw[s] = L.take(P[s].K * 8);
+        if (!B.plain && B.sp[s].Q == kChebQ) {
+            B.o_sf[s] = L.take(B.sp[s].strip_first.size() * 4);
+            B.o_sg[s] = L.take(B.sp[s].strip_geo.size() * 8 + 8);
+        }
         if (B.plain) B.o_tiles[s] = L.take(P[s].tiles.size() * sizeof(int4) + 16);
     }
     B.o_structs = L.take(ns * sizeof(DevStruct));
@@ -823,6 +896,12 @@ static void stage_build(const BuildPlan &B, const std::vector<Prepared> &P, char
         std::memcpy(hbase + B.o_kn[s], p.kind_n.data(), p.K * 8);
         std::memcpy(hbase + B.o_kw[s], p.kind_w.data(), p.K * 8);
         if (B.plain) std::memcpy(hbase + B.o_tiles[s], p.tiles.data(), p.tiles.size() * sizeof(int4));
+        const bool cheb = !B.plain && B.sp[s].Q == kChebQ;
+        if (cheb) {
+            const StructPlan &q = B.sp[s];
+            std::memcpy(hbase + B.o_sf[s], q.strip_first.data(), q.strip_first.size() * 4);
+            std::memcpy(hbase + B.o_sg[s], q.strip_geo.data(), q.strip_geo.size() * 8);
+        }
         DevStruct &d = ds[s];
         std::memset(&d, 0, sizeof(d));
         d.atoms = dbase + B.o_atoms[s];
@@ -843,7 +922,9 @@ static void stage_build(const BuildPlan &B, const std::vector<Prepared> &P, char
         d.small_h = p.small_h ? 1 : 0;
         d.sin_direct = p.sin_direct ? 1 : 0;
         d.n_classes = p.n_classes;
-        d.n_strips = B.plain ? 0 : (p.nq + B.sp[s].Q - 1) / B.sp[s].Q;
+        d.n_strips = B.plain ? 0 : B.sp[s].n_strips(p.nq);
+        d.strip_first = cheb ? (const int32_t *)(dbase + B.o_sf[s]) : nullptr;
+        d.strip_geo = cheb ? (const double *)(dbase + B.o_sg[s]) : nullptr;
         d.block = B.sp[s].block;
         for (int c = 0; c <= p.n_classes; ++c) d.cls_start[c] = p.cls_start[c];
         for (int c = p.n_classes + 1; c <= kMaxClasses; ++c) d.cls_start[c] = p.n;
@@ -874,6 +955,7 @@ static int make_launches(pripps_engine *e, const BuildPlan &B, char *dbase,
         LaunchRecord r;
         r.fp64 = B.fp64;
         r.Q = g.Q;
+        r.P = g.P;
         r.uniform = g.uniform;
         r.counter = (int *)(dbase + g.o_counter);
         r.n_items = (int64_t)g.items.size();
@@ -886,7 +968,8 @@ static int make_launches(pripps_engine *e, const BuildPlan &B, char *dbase,
         a.partials = (double *)(dbase + g.o_partials);
         a.variant = g.variant;
         int bpc = 1;
-        HIPCHK(tiled_occupancy(B.fp64, g.Q, g.uniform, g.variant, &bpc));
+        if (g.Q == kChebQ) HIPCHK(cheb_occupancy(g.P, &bpc));
+        else HIPCHK(tiled_occupancy(B.fp64, g.Q, g.uniform, g.variant, &bpc));
         bpc = std::max(1, bpc);
         const int64_t want = (r.n_items + 3) / 4;
         r.blocks = (int)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)e->n_cu * bpc));

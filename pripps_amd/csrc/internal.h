@@ -14,6 +14,10 @@ constexpr int kMaxKinds = 64;
 constexpr int kMaxClasses = 2 * kMaxKinds;
 constexpr int kProfiles = 6;  // DebyeTerms order: AA EE HH AE AH EH
 constexpr int kFinChunk = 32; // item partials summed per block in the first reduction stage
+// q-points per strip of the Chebyshev pair kernel (debye_cheb.hip); strips of
+// that kernel are runs of up to kChebQ consecutive q-points with their own
+// centre and half-width, on any grid.
+constexpr int kChebQ = 256;
 
 // One atom of the sorted SoA-in-AoS stream: centred coordinates and the
 // hydration weight, 16 bytes (FP32) or 32 bytes (FP64) per atom so a wave
@@ -42,6 +46,11 @@ struct DevStruct {
     int32_t block;          // j atoms summed in FP32 before the FP64 carry
     int32_t sin_direct;     // q_max r_max / 2 pi < 255 revolutions for every pair: the
                             // hardware sine takes the phase without a fract
+    // Chebyshev plans only (else nullptr): strip s covers q-points
+    // [strip_first[s], strip_first[s + 1]) with centre strip_geo[2 s] and
+    // half-width strip_geo[2 s + 1]
+    const int32_t *strip_first;
+    const double *strip_geo;
     int32_t cls_start[kMaxClasses + 1];
     int8_t cls_kind[kMaxClasses];
     int8_t cls_wet[kMaxClasses];
@@ -104,6 +113,14 @@ hipError_t launch_tiled(bool fp64, int Q, bool uniform, const TiledArgs &a, int 
                         hipStream_t st);
 hipError_t tiled_occupancy(bool fp64, int Q, bool uniform, int variant, int *blocks_per_cu);
 hipError_t launch_finalize(const FinalizeArgs &a, int n_strips_total, hipStream_t st);
+// Chebyshev pair kernel (debye_cheb.hip), P = expansion order (a built order)
+hipError_t launch_cheb(int P, const TiledArgs &a, int blocks, hipStream_t st);
+hipError_t cheb_occupancy(int P, int *blocks_per_cu);
+// smallest built order whose truncation is below 1e-9 of a pair's amplitude
+// at beta = H r up to beta_max (0: none)
+int cheb_order_for(double beta_max);
+// the largest beta_max that order P admits
+double cheb_beta_limit(int P);
 hipError_t launch_plain(const PlainArgs &a, int nq, hipStream_t st);
 hipError_t launch_fill_int(int *p, int n, int v, hipStream_t st);
 hipError_t launch_combine(const double *prof, int nq, double vs, double cd, double *out,

@@ -1,0 +1,95 @@
+"""GPU parity of the Chebyshev pair kernel (FP32, q-strips of up to 256
+points with a per-pair Jacobi-Anger expansion) against the CPU oracle, on
+uniform and non-uniform grids, with and without solvent, at the edges (empty
+and one-atom structures, coincident atoms, q = 0, one-point strips), and
+bitwise batch == alone.  Tolerance: FP32 compensated 1e-5 envelope error
+(test_gpu_parity.py)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import random_cluster
+from test_gpu_parity import build, envelope_error, make_case, oracle_profiles
+
+pytestmark = pytest.mark.gpu
+
+CHEB = 256
+
+
+def _cheb(**kw):
+    from pripps_amd._native import tuning
+    return tuning(q_per_lane=CHEB, **kw)
+
+
+@pytest.mark.parametrize("n,nq,qmax,solvent", [
+    (300, 37, 0.5, True), (700, 130, 0.5, True), (2000, 300, 0.5, True),
+    (1500, 600, 0.8, False), (2500, 500, 0.5, True), (64, 257, 0.5, True),
+    (65, 256, 0.5, False), (5, 9, 0.3, True), (2, 3, 0.5, True)])
+def test_cheb_uniform_matches_oracle(engine, oracle, n, nq, qmax, solvent):
+    case = make_case(n, 300 + n, nq=nq, qmax=qmax, solvent=solvent)
+    m = build(engine, case, "fp32", _cheb())
+    assert m.stats().q_per_lane == CHEB
+    err = envelope_error(m.profiles(), oracle_profiles(oracle, case))
+    assert err.max() <= 1e-5, err.max()
+
+
+@pytest.mark.parametrize("n,nq,lo,hi", [(1200, 400, 0.005, 1.0), (600, 50, 0.01, 0.6),
+                                        (900, 1000, 0.005, 1.0)])
+def test_cheb_log_grid_matches_oracle(engine, oracle, n, nq, lo, hi):
+    q = np.exp(np.linspace(np.log(lo), np.log(hi), nq))
+    case = make_case(n, 400 + n, q=q)
+    m = build(engine, case, "fp32", _cheb())
+    err = envelope_error(m.profiles(), oracle_profiles(oracle, case))
+    assert err.max() <= 1e-5, err.max()
+
+
+def test_cheb_unsorted_grid(engine, oracle):
+    """q values in no particular order: each strip's centre and half-width
+    come from its own minimum and maximum."""
+    rng = np.random.default_rng(5)
+    q = rng.permutation(np.linspace(0.0, 0.6, 300))
+    case = make_case(800, 77, q=q)
+    err = envelope_error(build(engine, case, "fp32", _cheb()).profiles(),
+                         oracle_profiles(oracle, case))
+    assert err.max() <= 1e-5, err.max()
+
+
+def test_cheb_edges(engine, oracle):
+    pos = random_cluster(50, 17)
+    pos = np.vstack([pos, pos[:5]])  # five duplicated atoms: r = 0, sinc = 1
+    rng = np.random.default_rng(3)
+    kind = rng.integers(0, 2, size=len(pos)).astype(np.int32)
+    for q in (np.array([0.0]), np.array([0.3]), np.array([0.0, 0.25]),
+              np.array([1e-4, 0.7, 0.1])):
+        ffa = np.array([[6.0 + k for _ in q] for k in range(2)])
+        case = (pos, kind, q, ffa, ffa * 0.4, np.full(len(q), 5.0), np.linspace(0, 1, len(pos)))
+        err = envelope_error(build(engine, case, "fp32", _cheb()).profiles(),
+                             oracle_profiles(oracle, case))
+        assert err.max() <= 1e-5, (q, err.max())
+    for n in (0, 1):
+        case = make_case(n, 3, nq=20)
+        got = build(engine, case, "fp32", _cheb()).profiles()
+        assert np.all(np.isfinite(got))
+        assert envelope_error(got, oracle_profiles(oracle, case)).max() <= 1e-5
+
+
+def test_cheb_many_kinds_and_blocks(engine, oracle):
+    case = make_case(900, 19, kinds=12, nq=200)
+    for block in (16, 64, 128):
+        got = build(engine, case, "fp32", _cheb(block=block)).profiles()
+        err = envelope_error(got, oracle_profiles(oracle, case))
+        assert err.max() <= 1e-5, (block, err.max())
+
+
+def test_cheb_batch_equals_single(engine):
+    """Structures of different sizes, grids and expansion orders in one
+    batch: each is bitwise the same as built alone."""
+    from pripps_amd._native import PreparedInput
+    cases = [make_case(n, 500 + n, nq=nq, qmax=qm) for n, nq, qm in
+             ((90, 41, 0.5), (1400, 300, 0.8), (5, 7, 0.3), (2600, 500, 0.45), (0, 10, 0.5),
+              (700, 640, 0.6))]
+    t = _cheb(block=64, span=256)
+    batch = engine.build_batch([PreparedInput(*c) for c in cases], "fp32", t)
+    for c, bm in zip(cases, batch):
+        assert np.array_equal(build(engine, c, "fp32", t).profiles(), bm.profiles())
