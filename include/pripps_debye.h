@@ -84,7 +84,9 @@ typedef struct pripps_debye_model pripps_debye_model;
 
 /* Kernel tuning; zero fields mean "choose automatically". */
 typedef struct {
-    int32_t q_per_lane;   /* q points per lane strip: 32, 64 or 128 (FP64: 32, 64) */
+    int32_t q_per_lane;   /* q points per lane strip: 32, 64 or 128 (FP64: 32, 64);
+                             256: the FP32 Chebyshev kernel (strips of up to 256
+                             q-points, any grid) */
     int32_t block;        /* pairs summed in FP32 before the FP64 carry */
     int32_t span;         /* atoms per work item along j */
     int32_t variant;      /* FP32 inner-loop variant for the autotuner (0 = default) */

@@ -588,9 +588,12 @@ void cheb_strips(const Prepared &p, StructPlan &sp) {
         for (int k = 0; k < 7; ++k) limit[k] = cheb_beta_limit(orders[k]);
     });
     const int nq = p.nq;
+    // the distance bound rounded up to a step of 2^(1/8): structures of about
+    // the same extent (jittered frames of one topology, say) get one plan
+    const double rplan = std::exp2(std::ceil(8.0 * std::log2(std::max(p.rmax, 1e-6))) / 8.0);
     double best_cost = 0;
     for (int k = 0; k < 7; ++k) {
-        const double hmax = limit[k] / std::max(p.rmax, 1e-300);
+        const double hmax = limit[k] / rplan;
         std::vector<int32_t> first{0};
         std::vector<double> geo;
         int m0 = 0;
@@ -631,10 +634,19 @@ void cheb_strips(const Prepared &p, StructPlan &sp) {
 // two plans measured even (1,200 atoms x 500 q; 64 was 16 % faster at 1,200 x
 // 200, 128 16 % faster at 1,200 x 800).  Non-uniform grids take a direct sine
 // per term and keep 32.
+//
+// FP32 from 3e7 pair-q terms on takes the Chebyshev kernel on any grid
+// (kChebQ).  Measured against the best recurrence plan (pair kernel, one
+// box): 1,000 atoms x 500 q 0.066 against 0.101 ms, 2,000 x 200 0.088
+// against 0.127, 10,000 x 500 0.634 against 1.706, 30,000 x 800 6.8 against
+// 22.9, and on a log grid 10,000 x 1,000 1.30 against 7.79 ms; only the
+// smallest structures (300 x 200: 0.077 against 0.054 ms) are faster with
+// the short recurrence strips.
 int default_q(int precision, bool uniform, int nq, int n_atoms) {
+    const double terms = 0.5 * (double)n_atoms * (double)n_atoms * (double)nq;
+    if (precision == PRIPPS_FP32_COMPENSATED && terms >= 3e7) return kChebQ;
     if (!uniform || nq <= 32) return 32;
     if (precision == PRIPPS_FP64) return n_atoms < 2048 ? 32 : 64;
-    const double terms = 0.5 * (double)n_atoms * (double)n_atoms * (double)nq;
     return (nq <= 64 || terms < 3e8) ? 64 : 128;
 }
 
@@ -1570,7 +1582,7 @@ extern "C" int pripps_engine_autotune(pripps_engine *e, const pripps_debye_input
     // 1. q points per lane
     {
         std::vector<Cand> c;
-        const int qs[3] = {32, 64, 128};
+        const int qs[4] = {32, 64, 128, kChebQ};
         for (int Q : qs)
             if (valid_q(precision, p.uniform, Q) == Q) c.push_back({Q, 0, cur.block, 3});
         if ((rc = try_all(c))) return rc;
@@ -1701,10 +1713,36 @@ extern "C" int pripps_debye_frames_average(pripps_engine *e, const pripps_debye_
         return rc;
     };
 
+    // Every chunk shares the first full chunk's plan (and the captured graph
+    // relies on that), and a Chebyshev plan's strips depend on the largest
+    // pair distance: plan every frame with one bound that holds for all used
+    // frames (twice the largest distance from a frame's centroid).
+    double rmax_all = 0;
+    for (int32_t f : used) {
+        const double *c = coords + (size_t)f * (size_t)n * 3;
+        double cx = 0, cy = 0, cz = 0;
+        for (int64_t i = 0; i < n; ++i) {
+            cx += c[3 * i];
+            cy += c[3 * i + 1];
+            cz += c[3 * i + 2];
+        }
+        if (n > 0) {
+            cx /= (double)n;
+            cy /= (double)n;
+            cz /= (double)n;
+        }
+        double r2 = 0;
+        for (int64_t i = 0; i < n; ++i) {
+            const double dx = c[3 * i] - cx, dy = c[3 * i + 1] - cy, dz = c[3 * i + 2] - cz;
+            r2 = std::max(r2, dx * dx + dy * dy + dz * dz);
+        }
+        rmax_all = std::max(rmax_all, 2.0 * std::sqrt(r2) * (1.0 + 1e-6) + 1e-6);
+    }
     // chunk size: frames per persistent launch, bounded by a 4 GiB workspace
     Prepared p0;
     int rc = prep_frame(used[0], p0);
     if (rc) return rc;
+    p0.rmax = std::max(p0.rmax, rmax_all);
     if (nq == 0 || p0.n == 0) return fail(PRIPPS_ERR_VALIDATION, "frames need atoms and q points");
     int C = M;
     {
@@ -1804,6 +1842,7 @@ extern "C" int pripps_debye_frames_average(pripps_engine *e, const pripps_debye_
                 in.y = by.data();
                 in.z = bz.data();
                 rcs[c] = prepare(&in, H.P[c]);
+                H.P[c].rmax = std::max(H.P[c].rmax, rmax_all);
                 if (rcs[c]) {
                     msgs[c] = "frame " + std::to_string(f) + ": " + g_last_error;  // this thread's
                 } else if (H.P[c].n != p0.n || H.P[c].nq != p0.nq) {

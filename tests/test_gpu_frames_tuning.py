@@ -98,7 +98,11 @@ def test_frames_graph_replays_match_per_frame_builds(engine, foxs, precision):
             assert s.frames_per_launch == 7 and s.graph_launches == 1
         prof = m.profiles()
         scale = np.abs(ref).max(axis=1, keepdims=True)
-        assert np.abs(prof - ref).max() <= 1e-13 * scale.max()
+        # FP64: the same plans, so only the averaging order differs.  FP32
+        # runs the Chebyshev kernel, whose strips follow a bound of the
+        # largest pair distance; the frames path takes one bound over all
+        # frames, so a frame's strips may differ from its own build's
+        assert np.abs(prof - ref).max() <= (1e-13 if precision == "fp64" else 1e-6) * scale.max()
         results.append(prof)
         # eager work between graph launches
         foxs.debye_model_from_arrays(frames[1], ids, **kw).profiles()
@@ -148,7 +152,7 @@ def test_autotune_table_applies_and_keeps_batches_deterministic(foxs):
     inps = [foxs.prepare_input(p, np.arange(len(p)) % 4, np.linspace(0, 0.5, 200))
             for p in names_pos]
     best, ms = eng.autotune(inps[0], "fp32", repeats=1)
-    assert ms > 0 and best.q_per_lane in (32, 64, 128) and best.span >= 64
+    assert ms > 0 and best.q_per_lane in (32, 64, 128, 256) and best.span >= 64
     table = eng.tuning_table()
     assert len(table) == 1
     t0 = table[0]

@@ -82,18 +82,16 @@ def test_nonuniform_q(engine, oracle, precision):
 
 
 @pytest.mark.parametrize("precision,n,nq,want_q", [
-    # FP32: 128-point strips from 3e8 pair-q terms (N^2 nq / 2) when nq > 64,
-    # else 64 (32 for nq <= 32)
-    ("fp32", 2000, 149, 64), ("fp32", 2000, 150, 128), ("fp32", 3100, 64, 64),
-    ("fp32", 3100, 65, 128), ("fp32", 2200, 129, 128), ("fp32", 2000, 192, 128),
-    ("fp32", 2000, 193, 128), ("fp32", 2048, 32, 32),
+    # FP32: the Chebyshev kernel (256) from 3e7 pair-q terms (N^2 nq / 2);
+    # below, 64-point strips when nq > 32, else 32
+    ("fp32", 500, 240, 256), ("fp32", 500, 239, 64), ("fp32", 1000, 61, 256),
+    ("fp32", 1000, 59, 64), ("fp32", 700, 20, 32), ("fp32", 2000, 150, 256),
+    ("fp32", 2048, 32, 256),
     # FP64: 64-point strips from 2,048 atoms when nq > 32, else 32
     ("fp64", 2047, 129, 32), ("fp64", 2048, 129, 64), ("fp64", 2048, 32, 32),
 ])
 def test_default_plan_boundaries(engine, oracle, precision, n, nq, want_q):
-    """The default strip length at its boundaries in N and nq, including grids
-    whose last 128-point strip has an empty second half (nq = 129, 192) or a
-    partly filled one (65, 193)."""
+    """The default kernel and strip length at their boundaries in N and nq."""
     case = make_case(n, 40 + nq, nq=nq)
     print(f"plan boundary n={n} nq={nq} precision={precision}", flush=True)
     m = build(engine, case, precision)
@@ -102,7 +100,8 @@ def test_default_plan_boundaries(engine, oracle, precision, n, nq, want_q):
     assert err.max() <= TOL[precision], (err.max(), np.unravel_index(err.argmax(), err.shape))
 
 
-@pytest.mark.parametrize("precision,qpl", [("fp32", 64), ("fp32", 128), ("fp64", 32), ("fp64", 64)])
+@pytest.mark.parametrize("precision,qpl", [("fp32", 64), ("fp32", 128), ("fp32", 256),
+                                           ("fp64", 32), ("fp64", 64)])
 def test_large_phases_and_coarse_grid(engine, oracle, precision, qpl):
     """q up to 6 1/A on a coarse grid over an 80 A wide cluster: phases q r / 2 pi
     up to ~80 revolutions (the seeds' exact reduction of large arguments), and a
