@@ -18,13 +18,16 @@
 //
 // Bessel weights per pair: Miller's backward recurrence on the scaled values
 // u_p = J_p(beta) (2 / beta)^p, u_{p-1} = p u_p - z u_{p+1} with z = (beta/2)^2,
-// started at N = P + 6 from (u_{N+1}, u_N) = (0, 2^-100).  The coefficients are
+// started at N = P + 2 from (u_{N+1}, u_N) = (0, 2^-100).  The coefficients are
 // exact integers, so no rounded constant enters the weights: their rounding
 // errors vary erratically from pair to pair instead of following r.  The scale
 // comes from Neumann's sum 1 = J_0 + 2 sum_m J_2m, which in u is
-// u_0 + 2 sum_m u_2m z^m (Horner along the recurrence).  Starting N - P orders
-// above the last one used leaves the unwanted (growing) solution below 1e-9
-// at order P for every beta the host admits.
+// u_0 + 2 sum_m u_2m z^m (Horner along the recurrence).  The unwanted
+// solution that the arbitrary start admits shrinks against the wanted one by
+// about z / p^2 per order on the way down; at the largest beta an order
+// admits (J_P below 5e-10 of the amplitude) z / P^2 is at most 0.04, so two
+// orders leave it under 2e-3 of J_P at order P, below 1e-12 of the pair's
+// amplitude, and smaller still at every lower order.
 //
 // The phases use the same split constants as the recurrence kernels: q_c / 2 pi
 // and H / 2 as float hi + lo parts, r and 1/r from residual-corrected square
@@ -70,7 +73,7 @@ __device__ __forceinline__ float wave_allsum(float v) {
 template <int P>
 __device__ __forceinline__ void cheb_pair(float r, float amp, float qc_hi, float qc_lo, float hh_hi,
                                           float hh_lo, float (&C)[P]) {
-    constexpr int N = P + 6;
+    constexpr int N = P + 2;
     const float arev = __builtin_fmaf(qc_hi, r, qc_lo * r);
     float sa, ca;
     sincos_rev(arev, sa, ca);
@@ -114,25 +117,42 @@ __device__ __forceinline__ void cheb_pair(float r, float amp, float qc_hi, float
 }
 
 // One chunk's pairs: j in [jb, je), all of one class.  BYW: amplitudes are
-// multiplied by w_j (the hydrated pass: sum w_j s).
+// multiplied by w_j (the hydrated pass: sum w_j s).  Even and odd j go to two
+// accumulator sets, added at the end: each FP32 partial sum then holds half
+// the chunk.  The Chebyshev form evaluates about P rounded coefficient sums
+// at every q-point, so their FP32 accumulation error weighs about sqrt(P)
+// times more than in a sum per q-point; halving the summation length
+// without halving the chunk keeps the carry's cost per pair.
 template <int P, bool MASK, bool BYW>
 __device__ __forceinline__ void cheb_block(const Atom<float> *__restrict__ atoms, int jb, int je,
                                            const Atom<float> &ai, int iidx, float lane_take,
                                            float qc_hi, float qc_lo, float hh_hi, float hh_lo,
                                            float (&C)[P]) {
     const PRIPPS_CONST Atom<float> *ca = as_const(atoms);
-    Atom<float> next = load_atom(ca, jb);
-    for (int j = jb; j < je; ++j) {
-        const Atom<float> aj = next;
-        next = load_atom(ca, min(j + 1, je - 1));  // prefetch the next j atom
+    float D[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) D[p] = 0.0f;
+    auto one = [&](const Atom<float> &aj, int j, float (&acc)[P]) {
         const float dx = ai.x - aj.x, dy = ai.y - aj.y, dz = ai.z - aj.z;
         const float r2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-12f);  // r -> 0: the limit s = q
         float r, rinv;
         sqrt_and_rsqrt(r2, r, rinv);
         float take = MASK ? (j > iidx ? 1.0f : 0.0f) : lane_take;
         if constexpr (BYW) take *= to_vgpr(aj.w);
-        cheb_pair<P>(r, take * rinv, qc_hi, qc_lo, hh_hi, hh_lo, C);
+        cheb_pair<P>(r, take * rinv, qc_hi, qc_lo, hh_hi, hh_lo, acc);
+    };
+    Atom<float> next = load_atom(ca, jb);
+    int j = jb;
+    for (; j + 1 < je; j += 2) {
+        const Atom<float> a0 = next;
+        const Atom<float> a1 = load_atom(ca, j + 1);
+        next = load_atom(ca, min(j + 2, je - 1));  // prefetch the next j atom
+        one(a0, j, C);
+        one(a1, j + 1, D);
     }
+    if (j < je) one(next, j, C);
+#pragma unroll
+    for (int p = 0; p < P; ++p) C[p] += D[p];
 }
 
 // One Clenshaw step in FP64 on an FP32 coefficient: b0 = 2x b1 - b2 + c.
