@@ -68,50 +68,67 @@ __device__ __forceinline__ float wave_allsum(float v) {
     return __builtin_bit_cast(float, (int)r32[0]) + __builtin_bit_cast(float, (int)r32[1]);
 }
 
-// Chebyshev coefficients of one pair added to C[0..P): see the file comment.
-// amp = take / r (0 for padding lanes and pairs on or below the diagonal).
-template <int P>
-__device__ __forceinline__ void cheb_pair(float r, float amp, float qc_hi, float qc_lo, float hh_hi,
-                                          float hh_lo, float (&C)[P]) {
+// Chebyshev coefficients of NP pairs, pair k added to its accumulator set
+// acc[k][0..P): see the file comment.  amp = take / r (0 for padding lanes
+// and pairs on or below the diagonal).
+template <int P, int NP>
+__device__ __forceinline__ void cheb_pairs(const float (&r)[NP], const float (&amp)[NP], float qc_hi,
+                                           float qc_lo, float hh_hi, float hh_lo,
+                                           float (&acc)[NP][P]) {
     constexpr int N = P + 2;
-    const float arev = __builtin_fmaf(qc_hi, r, qc_lo * r);
-    float sa, ca;
-    sincos_rev(arev, sa, ca);
-    const float b = __builtin_fmaf(hh_hi, r, hh_lo * r);  // beta / 2
-    const float z = b * b;
+    float sa[NP], ca[NP], b[NP], z[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        const float arev = __builtin_fmaf(qc_hi, r[k], qc_lo * r[k]);
+        sincos_rev(arev, sa[k], ca[k]);
+        b[k] = __builtin_fmaf(hh_hi, r[k], hh_lo * r[k]);  // beta / 2
+        z[k] = b[k] * b[k];
+    }
     // pass 1: u_{p-1} = p u_p - z u_{p+1}, downward from u_N
-    float u[P];
-    float un = 0.0f;
-    float uc = 7.888609052e-31f;  // 2^-100
-    float h = (N % 2 == 0) ? uc : 0.0f;
+    float u[NP][P];
+    float un[NP], uc[NP], h[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        un[k] = 0.0f;
+        uc[k] = 7.888609052e-31f;  // 2^-100
+        h[k] = (N % 2 == 0) ? uc[k] : 0.0f;
+    }
 #pragma unroll
     for (int p = N; p >= 1; --p) {
-        const float up = __builtin_fmaf(-z, un, float(p) * uc);
-        un = uc;
-        uc = up;
-        if (p - 1 < P) u[p - 1] = up;
-        if ((p - 1) % 2 == 0) h = __builtin_fmaf(h, z, up);
-    }
-    // Neumann normalisation: S = u_0 + 2 sum_m u_2m z^m = 2 h - u_0
-    const float S = __builtin_fmaf(2.0f, h, -u[0]);
-    float inv = __builtin_amdgcn_rcpf(S);
-    inv = __builtin_fmaf(inv, __builtin_fmaf(-S, inv, 1.0f), inv);
-    // pass 2: c_p = u_p W_p with W_0 = amp sin a / S, W_1 = 2 amp cos a b / S,
-    // W_2 = -2 z W_0 and W_{p+2} = -z W_p
-    const float a0 = amp * inv;
-    float we = a0 * sa;
-    float wo = 2.0f * ((a0 * ca) * b);
-    C[0] = __builtin_fmaf(u[0], we, C[0]);
-    if constexpr (P > 1) C[1] = __builtin_fmaf(u[1], wo, C[1]);
-    we = -2.0f * (z * we);
 #pragma unroll
-    for (int p = 2; p < P; ++p) {
-        if (p % 2 == 0) {
-            C[p] = __builtin_fmaf(u[p], we, C[p]);
-            we = -z * we;
-        } else {
-            wo = -z * wo;
-            C[p] = __builtin_fmaf(u[p], wo, C[p]);
+        for (int k = 0; k < NP; ++k) {
+            // -z u_{p+1} is known a step early: one FMA per order on the
+            // chain instead of a product and an FMA
+            const float up = __builtin_fmaf(float(p), uc[k], -z[k] * un[k]);
+            un[k] = uc[k];
+            uc[k] = up;
+            if (p - 1 < P) u[k][p - 1] = up;
+            if ((p - 1) % 2 == 0) h[k] = __builtin_fmaf(h[k], z[k], up);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+        // Neumann normalisation: S = u_0 + 2 sum_m u_2m z^m = 2 h - u_0
+        const float S = __builtin_fmaf(2.0f, h[k], -u[k][0]);
+        float inv = __builtin_amdgcn_rcpf(S);
+        inv = __builtin_fmaf(inv, __builtin_fmaf(-S, inv, 1.0f), inv);
+        // pass 2: c_p = u_p W_p with W_0 = amp sin a / S, W_1 = 2 amp cos a b / S,
+        // W_2 = -2 z W_0 and W_{p+2} = -z W_p
+        const float a0 = amp[k] * inv;
+        float we = a0 * sa[k];
+        float wo = 2.0f * ((a0 * ca[k]) * b[k]);
+        acc[k][0] = __builtin_fmaf(u[k][0], we, acc[k][0]);
+        if constexpr (P > 1) acc[k][1] = __builtin_fmaf(u[k][1], wo, acc[k][1]);
+        we = -2.0f * (z[k] * we);
+#pragma unroll
+        for (int p = 2; p < P; ++p) {
+            if (p % 2 == 0) {
+                acc[k][p] = __builtin_fmaf(u[k][p], we, acc[k][p]);
+                we = -z[k] * we;
+            } else {
+                wo = -z[k] * wo;
+                acc[k][p] = __builtin_fmaf(u[k][p], wo, acc[k][p]);
+            }
         }
     }
 }
@@ -129,17 +146,20 @@ __device__ __forceinline__ void cheb_block(const Atom<float> *__restrict__ atoms
                                            float qc_hi, float qc_lo, float hh_hi, float hh_lo,
                                            float (&C)[P]) {
     const PRIPPS_CONST Atom<float> *ca = as_const(atoms);
-    float D[P];
+    float acc[2][P];
 #pragma unroll
-    for (int p = 0; p < P; ++p) D[p] = 0.0f;
-    auto one = [&](const Atom<float> &aj, int j, float (&acc)[P]) {
+    for (int p = 0; p < P; ++p) {
+        acc[0][p] = C[p];
+        acc[1][p] = 0.0f;
+    }
+    auto geom = [&](const Atom<float> &aj, int j, float &r, float &amp) {
         const float dx = ai.x - aj.x, dy = ai.y - aj.y, dz = ai.z - aj.z;
         const float r2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-12f);  // r -> 0: the limit s = q
-        float r, rinv;
+        float rinv;
         sqrt_and_rsqrt(r2, r, rinv);
         float take = MASK ? (j > iidx ? 1.0f : 0.0f) : lane_take;
         if constexpr (BYW) take *= to_vgpr(aj.w);
-        cheb_pair<P>(r, take * rinv, qc_hi, qc_lo, hh_hi, hh_lo, acc);
+        amp = take * rinv;
     };
     Atom<float> next = load_atom(ca, jb);
     int j = jb;
@@ -147,12 +167,21 @@ __device__ __forceinline__ void cheb_block(const Atom<float> *__restrict__ atoms
         const Atom<float> a0 = next;
         const Atom<float> a1 = load_atom(ca, j + 1);
         next = load_atom(ca, min(j + 2, je - 1));  // prefetch the next j atom
-        one(a0, j, C);
-        one(a1, j + 1, D);
+        // one pair after the other: side by side their stored weights would
+        // take 2 P more VGPRs, beyond three waves per SIMD
+        float r[1], amp[1];
+        geom(a0, j, r[0], amp[0]);
+        cheb_pairs<P, 1>(r, amp, qc_hi, qc_lo, hh_hi, hh_lo, *reinterpret_cast<float(*)[1][P]>(&acc[0]));
+        geom(a1, j + 1, r[0], amp[0]);
+        cheb_pairs<P, 1>(r, amp, qc_hi, qc_lo, hh_hi, hh_lo, *reinterpret_cast<float(*)[1][P]>(&acc[1]));
     }
-    if (j < je) one(next, j, C);
+    if (j < je) {
+        float r[1], amp[1];
+        geom(next, j, r[0], amp[0]);
+        cheb_pairs<P, 1>(r, amp, qc_hi, qc_lo, hh_hi, hh_lo, *reinterpret_cast<float(*)[1][P]>(&acc[0]));
+    }
 #pragma unroll
-    for (int p = 0; p < P; ++p) C[p] += D[p];
+    for (int p = 0; p < P; ++p) C[p] = acc[0][p] + acc[1][p];
 }
 
 // One Clenshaw step in FP64 on an FP32 coefficient: b0 = 2x b1 - b2 + c.
@@ -182,6 +211,17 @@ __device__ __forceinline__ double cheb_eval(const float (&c)[P], double x) {
     }
     // c_0 + x b1 - b2
     return clenshaw_step(c[0], x, b1, b2);
+}
+
+// A wave-uniform pointer the compiler cannot see through: the carry's
+// per-lane addresses are then formed where they are used.  Hoisted out of
+// the chunk loop (they are invariant per item), the 64-bit addresses of the
+// q and form-factor loads of every q-point of the lane stay live through the
+// pair loop and cost some 60 VGPRs there.
+template <typename T>
+__device__ __forceinline__ const T *fresh(const T *p) {
+    asm volatile("" : "+s"(p));
+    return p;
 }
 
 template <int P>
@@ -267,15 +307,19 @@ __global__ void __launch_bounds__(256) debye_cheb_kernel(TiledArgs args) {
                 for (int t = 0; t < M; ++t) {
                     const int m = qf + 64 * t + lane;
                     if (m >= qend) continue;
-                    const double xt = (sd.q[m] - qc) * inv_h;
+                    const double *fq = fresh(sd.q);
+                    const double *fa = fresh(sd.ffa);
+                    const double *fe = sd.ffe ? fresh(sd.ffe) : nullptr;
+                    const double *fw = sd.ffw ? fresh(sd.ffw) : nullptr;
+                    const double xt = (fq[m] - qc) * inv_h;
                     const double s0 = cheb_eval<P>(C, xt);
                     const double s1 = wet_i ? cheb_eval<P>(ci, xt) : 0.0;
                     double *o = pr + 64 * t + lane;
-                    const double ak = sd.ffa[it.ikind * nq + m];
-                    const double ek = sd.ffe ? sd.ffe[it.ikind * nq + m] : 0.0;
-                    const double wq = sd.ffw ? sd.ffw[m] : 0.0;
-                    const double al = sd.ffa[lk * nq + m];
-                    const double el = sd.ffe ? sd.ffe[lk * nq + m] : 0.0;
+                    const double ak = fa[it.ikind * nq + m];
+                    const double ek = fe ? fe[it.ikind * nq + m] : 0.0;
+                    const double wq = fw ? fw[m] : 0.0;
+                    const double al = fa[lk * nq + m];
+                    const double el = fe ? fe[lk * nq + m] : 0.0;
                     if (pass == 0) {  // s0 = S, s1 = Si
                         o[0 * kChebQ] += 2.0 * ak * al * s0;
                         o[1 * kChebQ] += 2.0 * ek * el * s0;
