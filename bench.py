@@ -10,11 +10,17 @@ the finalize kernel over that structure, with inputs already in HBM.
 Reported beside the throughput:
   * precision: max envelope error of the FP32 profiles against the GPU FP64
     mode (all q) and against the CPU oracle (sampled q);
-  * roofline of the pair kernel: algorithmic FLOPs (14 per pair-q term with
-    solvent) over its HIP-event time, against the FP32 vector peak;
+  * roofline of the pair kernel: its executed FP32 FLOP rate (counter figures
+    of the same plan over the live HIP-event time) against the FP32 vector
+    peak, its VALU issue fraction, and the baseline's 14 FLOP per pair-q
+    term over the same time for reference;
   * the plain FP64 GPU path on the same structure (its pair kernel time);
   * cpu_baseline: the oracle's reference-order FP64 loop (3 self + 3 cross
-    sums per q, libm sin, OpenMP over q) on k evenly spaced q points.
+    sums per q, libm sin, OpenMP over q) on k evenly spaced q points;
+  * extra.batch512: the batched configuration (512 structures in one
+    persistent launch) against the plain FP64 path on a sample of them;
+  * served: Poisson request streams at three offered rates through the
+    serving loop, p50/p99 latency.
 """
 from __future__ import annotations
 
@@ -33,15 +39,28 @@ UNIT = "pair·q terms/s"
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
 FLOP_PER_TERM_SOLVENT = 14.0
-# Counter figures of the pair kernel on the default solvent10k plan (Q = 128,
-# span = 256, 12,928 items), from profiles/debye_dual_f32_solvent10k.md:
-#   bytes: HBM bytes per launch, rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
-#          (scale factors from a plain-copy calibration);
-#   executed_flop_per_term: rocprofv3 TOTAL_32_OPS ((2 FMA + ADD + MUL + TRANS +
-#          INT32) x 64 per dispatch) / pair-q terms.
-COUNTERS_SOLVENT10K = {"q_per_lane": 128, "span": 256, "work_items": 12928,
-                       "bytes": (2472.4 * 2.0 + 78036.0 * 1.0) * 1024,
-                       "executed_flop_per_term": 1.2833e11 / 2.49975e10}
+# Counter figures of the pair kernel on the default solvent10k plan, from one
+# rocprofv3 run each (profiles/debye_cheb_f32_solvent10k.md):
+#   bytes: HBM bytes per launch, FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (scale
+#          factors from a plain-copy calibration);
+#   executed_flop_per_term: FP32 FLOP per dispatch, (2 FMA + ADD + MUL + TRANS)
+#          x 64 lanes from SQ_INSTS_VALU_{FMA,ADD,MUL,TRANS}_F32, over the pass's
+#          pair-q terms;
+#   valu_wave_instr: SQ_INSTS_VALU per dispatch; with the dispatch's
+#          GRBM_GUI_ACTIVE cycles per XCD it gives the issue rate.
+COUNTERS = {
+    ("debye_cheb_kernel<28>", 256, 192, 8568): {
+        "bytes": (2134.0 * 2.0 + 103200.0 * 1.0) * 1024,
+        "executed_flop_per_term": 64.0 * (2 * 1.630e8 + 1.798e7 + 1.216e8 + 3.459e6) / 2.49975e10,
+        "valu_wave_instr": 3.534e8, "cycles_per_xcd": 9.677e6 / 8,
+        "profile": "profiles/debye_cheb_f32_solvent10k.md"},
+}
+F64_EXECUTED_FLOP_PER_TERM = 6.71
+F64_VALU_WAVE_INSTR = 2.299e9
+# VALU wave instructions a SIMD can issue per cycle (two per quad-cycle with
+# dual issue), and the engine clock
+VALU_ISSUE_PEAK = 0.5
+CLOCK_HZ = 2.4e9
 PAIRS = ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))
 
 
@@ -91,11 +110,19 @@ def main():
     ap.add_argument("--total-requests", type=int, default=0,
                     help="served: requests over the whole sweep, in proportion to the rate "
                          "(0: --requests per rate)")
-    ap.add_argument("--served-rate", type=float, default=200.0,
-                    help="default run: offered rate of its short served stream")
-    ap.add_argument("--served-requests", type=int, default=200,
-                    help="default run: requests in its short served stream (0: skip)")
+    ap.add_argument("--served-rates", default="100,400,1000",
+                    help="default run: offered rates of its served streams ('' skips them)")
+    ap.add_argument("--served-seconds", type=float, default=1.5,
+                    help="default run: length of each served stream (requests = rate x seconds)")
+    ap.add_argument("--served-requests", type=int, default=None,
+                    help="0 skips the default run's served streams (same as --served-rates '')")
+    ap.add_argument("--batch-count", type=int, default=512,
+                    help="default run: structures of its batched launch (0: skip)")
+    ap.add_argument("--batch-plain-sample", type=int, default=24,
+                    help="default run: batch structures timed on the sequential plain FP64 path")
     args = ap.parse_args()
+    if args.served_requests == 0:
+        args.served_rates = ""
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: the engine runs on one GPU and has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
@@ -138,11 +165,17 @@ def main():
         m64t.recompute(1)
         pm, _ = m64t.recompute(5)
         extra["fp64_tiled_pair_kernel_ms"] = pm
-        tf64 = FLOP_PER_TERM_SOLVENT * terms / (pm * 1e-3) / 1e12
-        extra["fp64_roofline"] = {"bound": "valu", "achieved": tf64, "peak": FP64_PEAK_TFLOPS,
-                                  "unit": "TFLOP/s", "frac": tf64 / FP64_PEAK_TFLOPS,
-                                  "q_per_lane": m64t.stats().q_per_lane,
-                                  "flop_per_term": FLOP_PER_TERM_SOLVENT}
+        # executed FP64 FLOP per term and VALU instructions per dispatch of the
+        # Q 64 plan from its counters (profiles/debye_tiled_f64_q64_solvent10k.md);
+        # FP64 instructions issue at most one per quad-cycle
+        tf64 = F64_EXECUTED_FLOP_PER_TERM * terms / (pm * 1e-3) / 1e12
+        extra["fp64_roofline"] = {
+            "bound": "valu", "achieved": tf64, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": tf64 / FP64_PEAK_TFLOPS, "q_per_lane": m64t.stats().q_per_lane,
+            "executed_flop_per_term": F64_EXECUTED_FLOP_PER_TERM,
+            "issue_frac": F64_VALU_WAVE_INSTR / (1024 * pm * 1e-3 * CLOCK_HZ) / 0.25,
+            "achieved_algorithmic": FLOP_PER_TERM_SOLVENT * terms / (pm * 1e-3) / 1e12,
+            "flop_per_term_algorithmic": FLOP_PER_TERM_SOLVENT}
         extra["fp64_tiled_vs_plain_maxerr"] = envelope_error(fp64_prof, plain_prof)
         log(f"fp64 tiled pair kernel {pm:.2f} ms; tiled vs plain {extra['fp64_tiled_vs_plain_maxerr']:.2e}")
         del m64, m64t
@@ -217,20 +250,39 @@ def main():
             log(f"cpu baseline, 1 core: {extra['cpu_baseline_1core']['value']:.3e} terms/s "
                 f"({t1c:.1f} s)")
 
-    plan = {"q_per_lane": st.q_per_lane, "span": st.span, "work_items": st.work_items}
-    same_plan = all(COUNTERS_SOLVENT10K[k] == v for k, v in plan.items())
-    traffic = COUNTERS_SOLVENT10K["bytes"] if same_plan else None
-    executed = None
-    if same_plan:
-        tf = COUNTERS_SOLVENT10K["executed_flop_per_term"] * terms / (pair_ms * 1e-3) / 1e12
-        executed = {"flop_per_term": COUNTERS_SOLVENT10K["executed_flop_per_term"],
-                    "achieved": tf, "frac": tf / FP32_PEAK_TFLOPS,
-                    "source": "rocprofv3 TOTAL_32_OPS of the same plan"}
+    kernel = (f"debye_cheb_kernel<{st.cheb_order}>" if st.q_per_lane == 256 else
+              "debye_dual_kernel" if st.q_per_lane == 128 else
+              "debye_tiled_kernel<float,%d,true,3,true>" % st.q_per_lane)
+    ctr = COUNTERS.get((kernel, st.q_per_lane, st.span, st.work_items))
+    roofline = {"bound": "valu", "unit": "TFLOP/s", "peak": FP32_PEAK_TFLOPS, "kernel": kernel,
+                "kernel_ms": pair_ms, "traffic": None, "achieved": None, "frac": None,
+                "issue_frac": None,
+                # the baseline's 14 FLOP per pair-q term (angle-addition
+                # count) over the same time: this kernel executes far fewer
+                "achieved_algorithmic": achieved,
+                "flop_per_term_algorithmic": FLOP_PER_TERM_SOLVENT}
+    if ctr is not None:
+        tf = ctr["executed_flop_per_term"] * terms / (pair_ms * 1e-3) / 1e12
+        roofline.update({
+            "achieved": tf, "frac": tf / FP32_PEAK_TFLOPS,
+            "executed_flop_per_term": ctr["executed_flop_per_term"],
+            "traffic": ctr["bytes"],
+            # issued VALU wave instructions per SIMD-cycle over the live
+            # kernel time at the engine clock, against two per quad-cycle
+            "issue_frac": ctr["valu_wave_instr"] / (1024 * pair_ms * 1e-3 * CLOCK_HZ) / VALU_ISSUE_PEAK,
+            "issue_frac_profiled": ctr["valu_wave_instr"] / 1024 / ctr["cycles_per_xcd"] / VALU_ISSUE_PEAK,
+            "valu_instr_per_term": ctr["valu_wave_instr"] * 64 / terms,
+            "source": ctr["profile"]})
+    log(f"roofline: {roofline}")
 
+    if args.batch_count > 0:
+        extra["batch512"] = _batch_line(eng, p, args)
     served = None
-    if args.served_requests > 0:
-        rec = _served_streams(eng, p, [args.served_rate], [args.served_requests])[0]
-        served = {"workload": SERVED_WORKLOAD, **rec}
+    if args.served_rates:
+        rates = [float(r) for r in args.served_rates.split(",")]
+        counts = [int(round(r * args.served_seconds)) for r in rates]
+        served = {"workload": SERVED_WORKLOAD, "seconds_per_rate": args.served_seconds,
+                  "rates": _served_streams(eng, p, rates, counts)}
 
     out = {
         "metric": METRIC,
@@ -249,13 +301,9 @@ def main():
                                "surface hydration weights, q uniform [0,0.5] nq=500, 6 profiles",
                    "n_atoms": wl.n_atoms, "nq": len(wl.q), "pair_q_terms": terms,
                    "precision": "fp32_compensated", "q_per_lane": st.q_per_lane,
-                   "block": st.block, "span": st.span, "work_items": st.work_items},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": ("debye_dual_kernel" if st.q_per_lane == 128
-                                else "debye_tiled_kernel<float,%d,true,3,true>" % st.q_per_lane),
-                     "kernel_ms": pair_ms, "flop_per_term": FLOP_PER_TERM_SOLVENT,
-                     "executed": executed},
+                   "cheb_order": st.cheb_order, "block": st.block, "span": st.span,
+                   "work_items": st.work_items},
+        "roofline": roofline,
         "cpu_baseline": cpu,
         "max_rel_err_vs_fp64_gpu": err_fp64,
         "max_rel_err_vs_oracle_sampled_q": err_oracle,
@@ -393,6 +441,42 @@ def _globule_job(job):
     from pripps_amd.synth import globule
     n, r, seed = job
     return globule(n, r, seed, solvent=True, jitter=0.5)
+
+
+def _batch_line(eng, ff, args):
+    """The batched configuration inside the default run: the structures in
+    one persistent FP32 launch (resident inputs, `recompute`), against an
+    evenly spaced sample of them built one at a time on the plain FP64 path
+    (the rate extrapolated by pair-q terms)."""
+    import numpy as np
+    t = time.perf_counter()
+    wls = _make_batch(args.batch_count)
+    inps = [ff.prepare_workload(w) for w in wls]
+    terms = float(sum(w.pair_q_terms for w in wls))
+    tb = time.perf_counter()
+    models = eng.build_batch(inps, "fp32")
+    first_wall = (time.perf_counter() - tb) * 1e3
+    ms, pair_ms = _timed(models[0], 4, 1)
+    del models
+    k = min(args.batch_plain_sample, len(wls))
+    sel = np.unique(np.linspace(0, len(wls) - 1, k).round().astype(int))
+    plain_ms, plain_terms = 0.0, 0.0
+    for s in sel:
+        m = eng.build(inps[s], "fp64_plain")
+        plain_ms += m.stats().kernel_ms
+        plain_terms += wls[s].pair_q_terms
+        del m
+    plain_rate = plain_terms / (plain_ms * 1e-3)
+    rec = {"workload": f"batch{len(wls)}: globules N log-uniform [500,20000], jitter 0.5 A, own q "
+                       "grid nq U[100,1000] qmax U[0.3,0.8], solvent; one persistent FP32 launch",
+           "structures": len(wls), "pair_q_terms": terms, "ms_per_step": ms, "pair_kernel_ms": pair_ms,
+           "terms_per_s": terms / (ms * 1e-3), "first_build_wall_ms": first_wall,
+           "fp64_plain_sequential": {"sample": int(len(sel)), "sample_terms": plain_terms,
+                                     "sample_kernel_ms": plain_ms, "terms_per_s": plain_rate},
+           "speedup_vs_sequential_plain": terms / (ms * 1e-3) / plain_rate,
+           "setup_s": tb - t}
+    log(f"batch: {rec}")
+    return rec
 
 
 def bench_batch512(args):

@@ -112,7 +112,7 @@ typedef struct {
     int32_t frames;              /* frames averaged (pripps_debye_frames_average), else 0 */
     int32_t frames_per_launch;   /* frames per persistent launch */
     int32_t graph_launches;      /* chunks submitted by replaying the captured HIP graph */
-    int32_t reserved;
+    int32_t cheb_order;          /* expansion order of a Chebyshev plan (q_per_lane 256), else 0 */
 } pripps_build_stats;
 
 /* Thread-local message of the last failed call on this thread ("" if none). */

@@ -65,7 +65,7 @@ class BuildStats(ctypes.Structure):
                 ("block", ctypes.c_int32), ("span", ctypes.c_int32),
                 ("work_items", ctypes.c_int64), ("frames", ctypes.c_int32),
                 ("frames_per_launch", ctypes.c_int32), ("graph_launches", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("cheb_order", ctypes.c_int32)]
 
 
 class TuningEntry(ctypes.Structure):

@@ -1166,6 +1166,7 @@ static int build_models(pripps_engine *e, int32_t ns, const pripps_debye_input *
         S.q_per_lane = B.plain ? 0 : B.sp[s].Q;
         S.block = B.plain ? 0 : B.sp[s].block;
         S.span = B.plain ? 0 : B.sp[s].span;
+        S.cheb_order = B.plain ? 0 : B.sp[s].P;
         S.work_items = B.n_items;
         out[s] = models[s];
     }
@@ -1986,6 +1987,7 @@ extern "C" int pripps_debye_frames_average(pripps_engine *e, const pripps_debye_
     S.frames = M;
     S.frames_per_launch = C;
     S.graph_launches = graph_launches;
+    S.cheb_order = sp0.P;
     *out = model;
     return PRIPPS_OK;
 }
