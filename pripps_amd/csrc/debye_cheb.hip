@@ -8,13 +8,20 @@
 //                               + cos a (2 sum_m (-1)^m J_2m+1(beta) T_2m+1(x))],
 // a Chebyshev series whose terms beyond order P are below 1e-9 of the pair's
 // amplitude (the host picks P from beta_max = H r_max with the bound
-// |J_p(beta)| <= (beta/2)^p / p!).  Each lane (one i atom) adds the P
-// coefficients of its pairs; once per chunk of up to `block` j atoms of one
-// class the wave sums them (DPP and lane-swap adds, bitwise the same in every
-// lane), and each lane evaluates the series at its own q-points in FP64
-// (Clenshaw) and carries the result into six FP64 profile sums, as the
-// recurrence kernels do.  Per pair and strip this costs about 60 + 4.5 P
-// instructions instead of three per q-point.
+// |J_p(beta)| <= (beta/2)^p / p!).
+//
+// Work item: a 64-atom i-tile (one lane per i atom) against a j-span, for a
+// group of one or two strips that share H; a wave takes a unit (a run of
+// items of one group) from the queue and sums it into the unit's slots.  Sharing H shares beta, so the
+// pair's Bessel weights -- most of the per-pair work -- serve both strips;
+// only sin a, cos a and the P accumulations are per strip.  Each lane adds
+// the coefficients of its pairs in FP32 over a chunk of up to `block` j
+// atoms of one class; then the wave reduce-scatters them (lane f receives
+// coefficient f summed over the 64 lanes) and adds them to FP64 sums in LDS.
+// When the j class changes, and at the end of the item, every lane evaluates
+// the FP64 series at its own q-points (Clenshaw), multiplies in the form
+// factors of the i kind and the j class, and adds the result to the item's
+// partial slot.  By linearity that equals summing the evaluated chunks.
 //
 // Bessel weights per pair: Miller's backward recurrence on the scaled values
 // u_p = J_p(beta) (2 / beta)^p, u_{p-1} = p u_p - z u_{p+1} with z = (beta/2)^2,
@@ -27,7 +34,10 @@
 // about z / p^2 per order on the way down; at the largest beta an order
 // admits (J_P below 5e-10 of the amplitude) z / P^2 is at most 0.04, so two
 // orders leave it under 2e-3 of J_P at order P, below 1e-12 of the pair's
-// amplitude, and smaller still at every lower order.
+// amplitude, and smaller still at every lower order.  The series
+// coefficients are then v_p = u_p * 2 (-z)^floor(p/2) times a per-strip
+// factor (amp sin a / S for even p, amp b cos a / S for odd p; p = 0 takes
+// half the even factor), with v_p formed once for both strips.
 //
 // The phases use the same split constants as the recurrence kernels: q_c / 2 pi
 // and H / 2 as float hi + lo parts, r and 1/r from residual-corrected square
@@ -41,306 +51,418 @@
 
 namespace pripps {
 
-constexpr int kChebM = kChebQ / 64;  // q-points per lane
+#define PRIPPS_LDS __attribute__((address_space(3)))
+#define PRIPPS_GLOBAL __attribute__((address_space(1)))
 
-// v + v of the lane DPP_CTRL names, fused into one v_add_f32_dpp.
+constexpr int kChebM = kChebQ / 64;  // q-points per lane and strip
+
+__device__ __forceinline__ int fbits(float v) { return __builtin_bit_cast(int, v); }
+__device__ __forceinline__ float bitsf(int v) { return __builtin_bit_cast(float, v); }
+
+// v as seen in the lane that DPP_CTRL names
 template <int DPP_CTRL>
-__device__ __forceinline__ float dpp_add(float v) {
-    const int o = __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), DPP_CTRL, 0xF, 0xF,
-                                              false);
-    return v + __builtin_bit_cast(float, o);
+__device__ __forceinline__ float dpp_from(float v) {
+    return bitsf(__builtin_amdgcn_update_dpp(0, fbits(v), DPP_CTRL, 0xF, 0xF, false));
 }
 
-// Sum over the 64 lanes, the same bits in every lane: quads (xor 1, xor 2),
-// half rows (mirror of 8), rows (mirror of 16), then row pairs and halves by
-// lane swaps.  Each step adds two operands that every participating lane sees
-// in the same order, so all lanes end with one identical value.
-__device__ __forceinline__ float wave_allsum(float v) {
-    v = dpp_add<0xB1>(v);   // quad_perm [1,0,3,2]
-    v = dpp_add<0x4E>(v);   // quad_perm [2,3,0,1]
-    v = dpp_add<0x141>(v);  // row_half_mirror
-    v = dpp_add<0x140>(v);  // row_mirror
-    const int b = __builtin_bit_cast(int, v);
-    auto r16 = __builtin_amdgcn_permlane16_swap(b, b, false, false);
-    v = __builtin_bit_cast(float, (int)r16[0]) + __builtin_bit_cast(float, (int)r16[1]);
-    const int c = __builtin_bit_cast(int, v);
-    auto r32 = __builtin_amdgcn_permlane32_swap(c, c, false, false);
-    return __builtin_bit_cast(float, (int)r32[0]) + __builtin_bit_cast(float, (int)r32[1]);
+// One in-row level of reduce_scatter: lanes with `hi` set keep value t + H of
+// each pair (t, t + H), the others value t, and each adds its DPP partner's
+// copy of the value it keeps.
+template <int DPP_CTRL, int H>
+__device__ __forceinline__ void rs_row_level(const float *in, float *out, bool hi) {
+#pragma unroll
+    for (int t = 0; t < H; ++t) {
+        const float keep = hi ? in[t + H] : in[t];
+        const float send = hi ? in[t] : in[t + H];
+        out[t] = keep + dpp_from<DPP_CTRL>(send);
+    }
 }
 
-// Chebyshev coefficients of NP pairs, pair k added to its accumulator set
-// acc[k][0..P): see the file comment.  amp = take / r (0 for padding lanes
-// and pairs on or below the diagonal).
-template <int P, int NP>
-__device__ __forceinline__ void cheb_pairs(const float (&r)[NP], const float (&amp)[NP], float qc_hi,
-                                           float qc_lo, float hh_hi, float hh_lo,
-                                           float (&acc)[NP][P]) {
+// Reduce-scatter of NV <= 64 per-lane values in(0..NV): lane L ends with the
+// sum over the 64 lanes of value L (NV <= 32: lanes L and L + 32 both hold
+// value L mod 32; lanes at or beyond NV hold sums of zeros).  Levels: wave
+// halves (permlane32 swap), row pairs (permlane16 swap), then inside each
+// 16-lane row the mirror (L ^ 15), the half-row mirror (L ^ 7), L ^ 2 and
+// L ^ 1 by DPP.  At each level a lane keeps the half of its values that the
+// level's lane bit selects and adds its partner's copy; the six lane masks
+// are independent, so each total covers every lane once, always in the same
+// order.
+template <int NV, typename F>
+__device__ __forceinline__ float reduce_scatter(F in) {
+    static_assert(NV >= 1 && NV <= 64, "one round holds up to 64 values");
+    const int lane = __lane_id();
+    float x[32];
+#pragma unroll
+    for (int t = 0; t < 32; ++t) {
+        if (NV > 32) {
+            const float a = in(t);
+            const float c = t + 32 < NV ? in(t + 32) : 0.0f;
+            const auto s = __builtin_amdgcn_permlane32_swap(fbits(a), fbits(c), false, false);
+            x[t] = bitsf(s[0]) + bitsf(s[1]);
+        } else if (t < NV) {
+            const float a = in(t);
+            const auto s = __builtin_amdgcn_permlane32_swap(fbits(a), fbits(a), false, false);
+            x[t] = bitsf(s[0]) + bitsf(s[1]);
+        } else {
+            x[t] = 0.0f;
+        }
+    }
+    float y[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        const auto s = __builtin_amdgcn_permlane16_swap(fbits(x[t]), fbits(x[t + 16]), false, false);
+        y[t] = bitsf(s[0]) + bitsf(s[1]);
+    }
+    float z[8], w[4], v[2];
+    rs_row_level<0x140, 8>(y, z, (lane & 8) != 0);  // row_mirror
+    rs_row_level<0x141, 4>(z, w, (lane & 4) != 0);  // row_half_mirror
+    rs_row_level<0x4E, 2>(w, v, (lane & 2) != 0);   // quad_perm [2,3,0,1]
+    float o[1];
+    rs_row_level<0xB1, 1>(v, o, (lane & 1) != 0);   // quad_perm [1,0,3,2]
+    return o[0];
+}
+
+// Coefficients of one pair for K strips of one group, added to acc[k][0..P):
+// see the file comment.  amp = take / r (0 for padding lanes and pairs on or
+// below the diagonal).
+template <int P, int K>
+__device__ __forceinline__ void cheb_pair(float r, float amp, const float (&qc_hi)[K],
+                                          const float (&qc_lo)[K], float hh_hi, float hh_lo,
+                                          float (&acc)[K][P]) {
     constexpr int N = P + 2;
-    float sa[NP], ca[NP], b[NP], z[NP];
+    float sa[K], ca[K];
 #pragma unroll
-    for (int k = 0; k < NP; ++k) {
-        const float arev = __builtin_fmaf(qc_hi, r[k], qc_lo * r[k]);
+    for (int k = 0; k < K; ++k) {
+        const float arev = __builtin_fmaf(qc_hi[k], r, qc_lo[k] * r);
         sincos_rev(arev, sa[k], ca[k]);
-        b[k] = __builtin_fmaf(hh_hi, r[k], hh_lo * r[k]);  // beta / 2
-        z[k] = b[k] * b[k];
     }
-    // pass 1: u_{p-1} = p u_p - z u_{p+1}, downward from u_N
-    float u[NP][P];
-    float un[NP], uc[NP], h[NP];
-#pragma unroll
-    for (int k = 0; k < NP; ++k) {
-        un[k] = 0.0f;
-        uc[k] = 7.888609052e-31f;  // 2^-100
-        h[k] = (N % 2 == 0) ? uc[k] : 0.0f;
-    }
+    const float b = __builtin_fmaf(hh_hi, r, hh_lo * r);  // beta / 2
+    const float z = b * b;
+    // pass 1: u_{p-1} = p u_p - z u_{p+1}, downward from u_N; -z u_{p+1} is
+    // known a step early, so the chain takes one FMA per order
+    float u[P];
+    float un = 0.0f, uc = 7.888609052e-31f;  // 2^-100
+    float h = (N % 2 == 0) ? uc : 0.0f;
 #pragma unroll
     for (int p = N; p >= 1; --p) {
-#pragma unroll
-        for (int k = 0; k < NP; ++k) {
-            // -z u_{p+1} is known a step early: one FMA per order on the
-            // chain instead of a product and an FMA
-            const float up = __builtin_fmaf(float(p), uc[k], -z[k] * un[k]);
-            un[k] = uc[k];
-            uc[k] = up;
-            if (p - 1 < P) u[k][p - 1] = up;
-            if ((p - 1) % 2 == 0) h[k] = __builtin_fmaf(h[k], z[k], up);
-        }
+        const float up = __builtin_fmaf(float(p), uc, -z * un);
+        un = uc;
+        uc = up;
+        if (p - 1 < P) u[p - 1] = up;
+        if ((p - 1) % 2 == 0) h = __builtin_fmaf(h, z, up);
     }
+    // Neumann normalisation: S = u_0 + 2 sum_m u_2m z^m = 2 h - u_0
+    const float S = __builtin_fmaf(2.0f, h, -u[0]);
+    float inv = __builtin_amdgcn_rcpf(S);
+    inv = __builtin_fmaf(inv, __builtin_fmaf(-S, inv, 1.0f), inv);
+    const float a0 = amp * inv;
+    const float a0b = a0 * b;
+    float e[K], o[K];
 #pragma unroll
-    for (int k = 0; k < NP; ++k) {
-        // Neumann normalisation: S = u_0 + 2 sum_m u_2m z^m = 2 h - u_0
-        const float S = __builtin_fmaf(2.0f, h[k], -u[k][0]);
-        float inv = __builtin_amdgcn_rcpf(S);
-        inv = __builtin_fmaf(inv, __builtin_fmaf(-S, inv, 1.0f), inv);
-        // pass 2: c_p = u_p W_p with W_0 = amp sin a / S, W_1 = 2 amp cos a b / S,
-        // W_2 = -2 z W_0 and W_{p+2} = -z W_p
-        const float a0 = amp[k] * inv;
-        float we = a0 * sa[k];
-        float wo = 2.0f * ((a0 * ca[k]) * b[k]);
-        acc[k][0] = __builtin_fmaf(u[k][0], we, acc[k][0]);
-        if constexpr (P > 1) acc[k][1] = __builtin_fmaf(u[k][1], wo, acc[k][1]);
-        we = -2.0f * (z[k] * we);
+    for (int k = 0; k < K; ++k) {
+        e[k] = a0 * sa[k];   // even orders: amp sin a / S
+        o[k] = a0b * ca[k];  // odd orders: amp (beta/2) cos a / S
+    }
+    // pass 2: c_0 = u_0 e, c_p = u_p zeta_m (e or o) with zeta_m = 2 (-z)^m,
+    // m = floor(p / 2)
 #pragma unroll
-        for (int p = 2; p < P; ++p) {
-            if (p % 2 == 0) {
-                acc[k][p] = __builtin_fmaf(u[k][p], we, acc[k][p]);
-                we = -z[k] * we;
-            } else {
-                wo = -z[k] * wo;
-                acc[k][p] = __builtin_fmaf(u[k][p], wo, acc[k][p]);
-            }
-        }
+    for (int k = 0; k < K; ++k) acc[k][0] = __builtin_fmaf(u[0], e[k], acc[k][0]);
+    float zeta = 2.0f;
+#pragma unroll
+    for (int p = 1; p < P; ++p) {
+        if (p >= 2 && p % 2 == 0) zeta = -z * zeta;
+        const float v = u[p] * zeta;
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            acc[k][p] = __builtin_fmaf(v, (p % 2 == 0) ? e[k] : o[k], acc[k][p]);
     }
 }
 
-// One chunk's pairs: j in [jb, je), all of one class.  BYW: amplitudes are
-// multiplied by w_j (the hydrated pass: sum w_j s).  Even and odd j go to two
-// accumulator sets, added at the end: each FP32 partial sum then holds half
-// the chunk.  The Chebyshev form evaluates about P rounded coefficient sums
-// at every q-point, so their FP32 accumulation error weighs about sqrt(P)
-// times more than in a sum per q-point; halving the summation length
-// without halving the chunk keeps the carry's cost per pair.
-template <int P, bool MASK, bool BYW>
-__device__ __forceinline__ void cheb_block(const Atom<float> *__restrict__ atoms, int jb, int je,
+// One chunk's pairs: j in [jb, je), all of one class, summed per lane into
+// acc (cleared first).  BYW: amplitudes are multiplied by w_j (the hydrated
+// pass: sum w_j s).
+template <int P, int K, bool MASK, bool BYW>
+__device__ __forceinline__ void cheb_chunk(const Atom<float> *__restrict__ atoms, int jb, int je,
                                            const Atom<float> &ai, int iidx, float lane_take,
-                                           float qc_hi, float qc_lo, float hh_hi, float hh_lo,
-                                           float (&C)[P]) {
+                                           const float (&qc_hi)[K], const float (&qc_lo)[K],
+                                           float hh_hi, float hh_lo, float (&acc)[K][P]) {
     const PRIPPS_CONST Atom<float> *ca = as_const(atoms);
-    float acc[2][P];
 #pragma unroll
-    for (int p = 0; p < P; ++p) {
-        acc[0][p] = C[p];
-        acc[1][p] = 0.0f;
-    }
-    auto geom = [&](const Atom<float> &aj, int j, float &r, float &amp) {
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int p = 0; p < P; ++p) acc[k][p] = 0.0f;
+    auto pair = [&](const Atom<float> &aj, int j) {
         const float dx = ai.x - aj.x, dy = ai.y - aj.y, dz = ai.z - aj.z;
         const float r2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-12f);  // r -> 0: the limit s = q
-        float rinv;
+        float r, rinv;
         sqrt_and_rsqrt(r2, r, rinv);
         float take = MASK ? (j > iidx ? 1.0f : 0.0f) : lane_take;
         if constexpr (BYW) take *= to_vgpr(aj.w);
-        amp = take * rinv;
+        cheb_pair<P, K>(r, take * rinv, qc_hi, qc_lo, hh_hi, hh_lo, acc);
     };
-    Atom<float> next = load_atom(ca, jb);
+    // Two pairs per iteration, each atom's scalar load issued one pair ahead
+    // into its own registers.  The empty asm that reads an atom at the end of
+    // the pair before its use keeps its load there: left to itself the
+    // compiler issues the load at the top of the next iteration (or copies
+    // the loaded registers early) and waits a full cache latency per pair.
+    // The pairs are not interleaved: at three waves per SIMD two interleaved
+    // pairs issue slower than one pair after the other (measured).
+    Atom<float> a0 = load_atom(ca, jb);
+    asm volatile("" ::"s"(a0.x), "s"(a0.y), "s"(a0.z), "s"(a0.w));  // waited for here, once
     int j = jb;
     for (; j + 1 < je; j += 2) {
-        const Atom<float> a0 = next;
+        // (scheduling barriers keep each load ahead of the pair and each
+        // empty asm after it, and the two pairs apart)
         const Atom<float> a1 = load_atom(ca, j + 1);
-        next = load_atom(ca, min(j + 2, je - 1));  // prefetch the next j atom
-        // one pair after the other: side by side their stored weights would
-        // take 2 P more VGPRs, beyond three waves per SIMD
-        float r[1], amp[1];
-        geom(a0, j, r[0], amp[0]);
-        cheb_pairs<P, 1>(r, amp, qc_hi, qc_lo, hh_hi, hh_lo, *reinterpret_cast<float(*)[1][P]>(&acc[0]));
-        geom(a1, j + 1, r[0], amp[0]);
-        cheb_pairs<P, 1>(r, amp, qc_hi, qc_lo, hh_hi, hh_lo, *reinterpret_cast<float(*)[1][P]>(&acc[1]));
+        __builtin_amdgcn_sched_barrier(0);
+        pair(a0, j);
+        __builtin_amdgcn_sched_barrier(0);
+        asm volatile("" ::"s"(a1.x), "s"(a1.y), "s"(a1.z), "s"(a1.w));
+        a0 = load_atom(ca, min(j + 2, je - 1));
+        __builtin_amdgcn_sched_barrier(0);
+        pair(a1, j + 1);
+        __builtin_amdgcn_sched_barrier(0);
+        asm volatile("" ::"s"(a0.x), "s"(a0.y), "s"(a0.z), "s"(a0.w));
     }
-    if (j < je) {
-        float r[1], amp[1];
-        geom(next, j, r[0], amp[0]);
-        cheb_pairs<P, 1>(r, amp, qc_hi, qc_lo, hh_hi, hh_lo, *reinterpret_cast<float(*)[1][P]>(&acc[0]));
+    if (j < je) pair(a0, j);
+}
+
+// acc summed over the wave into the FP64 sums of set `set` (and w_i acc into
+// set + 1 when wi_on), lane-distributed: coefficient f = k P + p of round
+// f / 64 lives at d[set * RS + f].  The w_i-weighted sums go first, on
+// products; the plain sums last, so that their lane swaps can work on the
+// accumulators in place once nothing else reads them.
+template <int P, int K, int RS>
+__device__ __forceinline__ void cheb_carry(const float (&acc)[K][P], PRIPPS_LDS double *d, int set,
+                                           bool wi_on, float wi) {
+    constexpr int NV = K * P;
+    constexpr int n0 = NV < 64 ? NV : 64;
+    constexpr int n1 = NV > 64 ? NV - 64 : 1;
+    static_assert(NV <= 128, "two rounds at most");
+    const int lane = __lane_id();
+    auto flat = [&](int f) { return acc[f / P][f % P]; };
+    if (wi_on) {
+        float si = reduce_scatter<n0>([&](int t) { return flat(t) * wi; });
+        d[(set + 1) * RS + lane] += (double)si;
+        if constexpr (NV > 64) {
+            si = reduce_scatter<n1>([&](int t) { return flat(64 + t) * wi; });
+            d[(set + 1) * RS + 64 + lane] += (double)si;
+        }
     }
+    float s = reduce_scatter<n0>([&](int t) { return flat(t); });
+    d[set * RS + lane] += (double)s;
+    if constexpr (NV > 64) {
+        s = reduce_scatter<n1>([&](int t) { return flat(64 + t); });
+        d[set * RS + 64 + lane] += (double)s;
+    }
+}
+
+// End of a class run: the wave's FP64 coefficient sums (every set, zero
+// where unused) go to record `rec` as they lie in LDS, one coefficient per
+// lane, and the LDS sums restart at zero.
+template <int RS>
+__device__ __forceinline__ void cheb_record(PRIPPS_LDS double *d, double *records, int rec) {
+    const int lane = __lane_id();
+    double *o = records + (int64_t)rec * (kChebSets * RS) + lane;
 #pragma unroll
-    for (int p = 0; p < P; ++p) C[p] = acc[0][p] + acc[1][p];
-}
-
-// One Clenshaw step in FP64 on an FP32 coefficient: b0 = 2x b1 - b2 + c.
-// The conversion of c is bound to its use in one asm block; left to itself
-// the compiler converts all P coefficients ahead of the recursion and holds
-// their FP64 copies (2 P VGPRs) through the carry.
-__device__ __forceinline__ double clenshaw_step(float c, double x2, double b1, double b2) {
-    double b0, t;
-    asm("v_cvt_f64_f32 %[t], %[c]\n\t"
-        "v_add_f64 %[t], %[t], -%[b2]\n\t"
-        "v_fma_f64 %[b0], %[x2], %[b1], %[t]"
-        : [b0] "=&v"(b0), [t] "=&v"(t)
-        : [c] "v"(c), [x2] "v"(x2), [b1] "v"(b1), [b2] "v"(b2));
-    return b0;
-}
-
-// sum_p c_p T_p(x) (Clenshaw in FP64 on FP32 coefficients)
-template <int P>
-__device__ __forceinline__ double cheb_eval(const float (&c)[P], double x) {
-    double b1 = 0.0, b2 = 0.0;
-    const double x2 = 2.0 * x;
+    for (int s = 0; s < kChebSets; ++s)
 #pragma unroll
-    for (int p = P - 1; p >= 1; --p) {
-        const double b0 = clenshaw_step(c[p], x2, b1, b2);
-        b2 = b1;
-        b1 = b0;
-    }
-    // c_0 + x b1 - b2
-    return clenshaw_step(c[0], x, b1, b2);
+        for (int r = 0; r < RS; r += 64) {
+            o[s * RS + r] = d[s * RS + r + lane];
+            d[s * RS + r + lane] = 0.0;
+        }
 }
 
-// A wave-uniform pointer the compiler cannot see through: the carry's
-// per-lane addresses are then formed where they are used.  Hoisted out of
-// the chunk loop (they are invariant per item), the 64-bit addresses of the
-// q and form-factor loads of every q-point of the lane stay live through the
-// pair loop and cost some 60 VGPRs there.
-template <typename T>
-__device__ __forceinline__ const T *fresh(const T *p) {
-    asm volatile("" : "+s"(p));
-    return p;
+template <int P, int K, int RS>
+__device__ __forceinline__ void cheb_item(const TiledArgs &args, const Item &it, PRIPPS_LDS double *d,
+                                          int &rec) {
+    const int lane = __lane_id();
+    const PRIPPS_CONST DevStruct &sd = as_const(args.structs)[it.s];
+    const Atom<float> *__restrict__ atoms = static_cast<const Atom<float> *>(sd.atoms);
+
+    const bool ivalid = lane < it.icount;
+    const int iidx = ivalid ? it.i0 + lane : 0x7fffffff;
+    const Atom<float> ai = atoms[ivalid ? it.i0 + lane : it.i0];
+    const float wi = ivalid ? ai.w : 0.0f;
+    const float lane_take = ivalid ? 1.0f : 0.0f;
+    const bool wet_i = __ballot(wi != 0.0f) != 0;
+
+    // phases of the strips' centres and the group's half-width (shared)
+    float qc_hi[K], qc_lo[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double qcr = sd.strip_geo[2 * (it.strip + k)] * kInv2Pi;
+        qc_hi[k] = to_vgpr(float(qcr));
+        qc_lo[k] = to_vgpr(float(qcr - double(float(qcr))));
+    }
+    const double hh = 0.5 * sd.strip_geo[2 * it.strip + 1];
+    const float hh_hi = to_vgpr(float(hh));
+    const float hh_lo = to_vgpr(float(hh - double(float(hh))));
+
+    int j = it.j0;
+    int c = 0;
+    int cur = -1;
+    while (j < it.j1) {
+        while (sd.cls_start[c + 1] <= j) ++c;
+        if (cur >= 0 && c != cur) cheb_record<RS>(d, args.partials, rec++);
+        cur = c;
+        const int jend = min(min(j + sd.block, it.j1), sd.cls_start[c + 1]);
+        const bool wet = sd.cls_wet[c] != 0;
+        const bool mask = j <= it.i0 + it.icount - 1;  // chunk meets the diagonal
+        // pass 0: sum s (and w_i s); pass 1 (hydrated j only): sum w_j s (and w_i w_j s)
+#pragma unroll
+        for (int pass = 0; pass < 2; ++pass) {
+            if (pass == 1 && !wet) break;
+            float acc[K][P];
+            if (pass == 0) {
+                if (mask) cheb_chunk<P, K, true, false>(atoms, j, jend, ai, iidx, lane_take, qc_hi, qc_lo, hh_hi, hh_lo, acc);
+                else cheb_chunk<P, K, false, false>(atoms, j, jend, ai, iidx, lane_take, qc_hi, qc_lo, hh_hi, hh_lo, acc);
+            } else {
+                if (mask) cheb_chunk<P, K, true, true>(atoms, j, jend, ai, iidx, lane_take, qc_hi, qc_lo, hh_hi, hh_lo, acc);
+                else cheb_chunk<P, K, false, true>(atoms, j, jend, ai, iidx, lane_take, qc_hi, qc_lo, hh_hi, hh_lo, acc);
+            }
+            cheb_carry<P, K, RS>(acc, d, 2 * pass, wet_i, wi);
+        }
+        j = jend;
+    }
+    cheb_record<RS>(d, args.partials, rec++);
 }
 
 template <int P>
 __global__ void __launch_bounds__(256) debye_cheb_kernel(TiledArgs args) {
-    constexpr int M = kChebM;
+    // per wave: the FP64 coefficient sums of the current class run, four sets
+    // of up to two rounds of 64 (the two strips' K P coefficients)
+    constexpr int RS = (2 * P + 63) / 64 * 64;
+    __shared__ double coef[4][kChebSets * RS];
+    PRIPPS_LDS double *const d =
+        (PRIPPS_LDS double *)coef[__builtin_amdgcn_readfirstlane(threadIdx.x / 64)];
     const int lane = __lane_id();
-    // the wave's FP64 profile sums [profile][q-local]; each lane touches only
-    // its own M columns (lane + 64 t), so no barrier is needed.  In VGPRs they
-    // would take 48 registers from the pair loop's occupancy.
-    __shared__ double sums[4][kProfiles * kChebQ];
-    double *const pr = sums[threadIdx.x / 64];
+#pragma unroll
+    for (int s = 0; s < kChebSets; ++s)
+#pragma unroll
+        for (int r = 0; r < RS; r += 64) d[s * RS + r + lane] = 0.0;
     for (;;) {
         int w = 0;
         if (lane == 0) w = atomicAdd(args.counter, 1);
         w = __builtin_amdgcn_readfirstlane(__shfl(w, 0, 64));
         if (w >= args.n_items) break;
-
-        const Item it = load_item(args.items, w);
-        const PRIPPS_CONST DevStruct &sd = as_const(args.structs)[it.s];
-        const Atom<float> *__restrict__ atoms = static_cast<const Atom<float> *>(sd.atoms);
-        const int nq = sd.nq;
-
-        const bool ivalid = lane < it.icount;
-        const int iidx = ivalid ? it.i0 + lane : 0x7fffffff;
-        const Atom<float> ai = atoms[ivalid ? it.i0 + lane : it.i0];
-        const float wi = ivalid ? ai.w : 0.0f;
-        const float lane_take = ivalid ? 1.0f : 0.0f;
-        const bool wet_i = __ballot(wi != 0.0f) != 0;
-
-        // strip geometry
-        const int qf = sd.strip_first[it.strip];
-        const int qend = sd.strip_first[it.strip + 1];
-        const double qc = sd.strip_geo[2 * it.strip];
-        const double H = sd.strip_geo[2 * it.strip + 1];
-        const double qcr = qc * kInv2Pi, hh = 0.5 * H;
-        const float qc_hi = to_vgpr(float(qcr));
-        const float qc_lo = to_vgpr(float(qcr - double(float(qcr))));
-        const float hh_hi = to_vgpr(float(hh));
-        const float hh_lo = to_vgpr(float(hh - double(float(hh))));
-        // the lane's q-points m = qf + 64 t + lane, at x = (q_m - q_c) / H
-        const double inv_h = H > 0.0 ? 1.0 / H : 0.0;
-#pragma unroll
-        for (int k = 0; k < kProfiles; ++k)
-#pragma unroll
-            for (int t = 0; t < M; ++t) pr[k * kChebQ + 64 * t + lane] = 0.0;
-
-        int j = it.j0;
-        int c = 0;
-        while (j < it.j1) {
-            while (sd.cls_start[c + 1] <= j) ++c;
-            const int jend = min(min(j + sd.block, it.j1), sd.cls_start[c + 1]);
-            const int lk = sd.cls_kind[c];
-            const bool wet = sd.cls_wet[c] != 0;
-            const bool mask = j <= it.i0 + it.icount - 1;  // chunk meets the diagonal
-
-            // pass 0: S = sum s and Si = sum w_i s; pass 1 (hydrated j only):
-            // Sw = sum w_j s and Siw = sum w_i w_j s -- at the lane's q-points,
-            // each carried into the FP64 profile sums as soon as it is known
-#pragma unroll
-            for (int pass = 0; pass < 2; ++pass) {
-                if (pass == 1 && !wet) break;
-                float C[P];
-#pragma unroll
-                for (int p = 0; p < P; ++p) C[p] = 0.0f;
-                if (pass == 0) {
-                    if (mask) cheb_block<P, true, false>(atoms, j, jend, ai, iidx, lane_take, qc_hi, qc_lo, hh_hi, hh_lo, C);
-                    else cheb_block<P, false, false>(atoms, j, jend, ai, iidx, lane_take, qc_hi, qc_lo, hh_hi, hh_lo, C);
-                } else {
-                    if (mask) cheb_block<P, true, true>(atoms, j, jend, ai, iidx, lane_take, qc_hi, qc_lo, hh_hi, hh_lo, C);
-                    else cheb_block<P, false, true>(atoms, j, jend, ai, iidx, lane_take, qc_hi, qc_lo, hh_hi, hh_lo, C);
-                }
-                // wave sums (the w_i-weighted ones first, then C in place),
-                // then one q-point at a time: the series at x, the form
-                // factors, the carry into the LDS profile sums
-                float ci[P];
-                if (wet_i) {
-#pragma unroll
-                    for (int p = 0; p < P; ++p) ci[p] = wave_allsum(C[p] * wi);
-                }
-#pragma unroll
-                for (int p = 0; p < P; ++p) C[p] = wave_allsum(C[p]);
-#pragma unroll 1
-                for (int t = 0; t < M; ++t) {
-                    const int m = qf + 64 * t + lane;
-                    if (m >= qend) continue;
-                    const double *fq = fresh(sd.q);
-                    const double *fa = fresh(sd.ffa);
-                    const double *fe = sd.ffe ? fresh(sd.ffe) : nullptr;
-                    const double *fw = sd.ffw ? fresh(sd.ffw) : nullptr;
-                    const double xt = (fq[m] - qc) * inv_h;
-                    const double s0 = cheb_eval<P>(C, xt);
-                    const double s1 = wet_i ? cheb_eval<P>(ci, xt) : 0.0;
-                    double *o = pr + 64 * t + lane;
-                    const double ak = fa[it.ikind * nq + m];
-                    const double ek = fe ? fe[it.ikind * nq + m] : 0.0;
-                    const double wq = fw ? fw[m] : 0.0;
-                    const double al = fa[lk * nq + m];
-                    const double el = fe ? fe[lk * nq + m] : 0.0;
-                    if (pass == 0) {  // s0 = S, s1 = Si
-                        o[0 * kChebQ] += 2.0 * ak * al * s0;
-                        o[1 * kChebQ] += 2.0 * ek * el * s0;
-                        o[3 * kChebQ] += (ak * el + al * ek) * s0;
-                        o[4 * kChebQ] += wq * (al * s1);
-                        o[5 * kChebQ] += wq * (el * s1);
-                    } else {          // s0 = Sw, s1 = Siw
-                        o[2 * kChebQ] += 2.0 * wq * wq * s1;
-                        o[4 * kChebQ] += wq * (ak * s0);
-                        o[5 * kChebQ] += wq * (ek * s0);
-                    }
-                }
-            }
-            j = jend;
+        const PRIPPS_CONST Unit &un = as_const(args.units)[w];
+        const int i0 = un.item0, i1 = un.item1, nk = un.nk;
+        int rec = un.rec0;
+        for (int i = i0; i < i1; ++i) {
+            const Item it = load_item(args.items, i);
+            if (nk == 2) cheb_item<P, 2, RS>(args, it, d, rec);
+            else cheb_item<P, 1, RS>(args, it, d, rec);
         }
-        double *out = args.partials + (int64_t)w * (kProfiles * kChebQ);
-#pragma unroll
-        for (int k = 0; k < kProfiles; ++k)
-#pragma unroll
-            for (int t = 0; t < M; ++t) out[k * kChebQ + 64 * t + lane] = pr[k * kChebQ + 64 * t + lane];
     }
+}
+
+// ---------------------------------------------------------------------------
+// reduction of the coefficient records
+// ---------------------------------------------------------------------------
+// Stage 1: block c sums the records of chunk c (up to kFinChunk records of one
+// combo) in list order; thread t takes coefficients t, t + 256, ...
+__global__ void __launch_bounds__(256) cheb_chunk_kernel(ChebReduceArgs a) {
+    const int c = blockIdx.x;
+    const int b = a.chunk_first[c], e = a.chunk_first[c + 1];
+    const int n = kChebSets * a.RS;
+    for (int t = threadIdx.x; t < n; t += 256) {
+        double acc = 0.0;
+#pragma unroll 4
+        for (int i = b; i < e; ++i) acc += a.records[(int64_t)a.chunk_recs[i] * n + t];
+        a.chunk_tot[(int64_t)c * n + t] = acc;
+    }
+}
+
+// Stage 2: block b sums its combo's chunk totals (in order) into LDS, then
+// thread ql evaluates the four series at q-point m = first + ql of its strip
+// (Clenshaw in FP64, coefficients k P .. k P + P - 1) and writes the six
+// profile contributions of the combo's form factors.
+__global__ void __launch_bounds__(kChebQ) cheb_combo_kernel(ChebReduceArgs a) {
+    __shared__ double coef[kChebSets * 128];
+    const int4 sa = a.sb_a[blockIdx.x], sb = a.sb_b[blockIdx.x];
+    const int n = kChebSets * a.RS;
+    for (int t = threadIdx.x; t < n; t += kChebQ) {
+        double acc = 0.0;
+        for (int c = sb.y; c < sb.z; ++c) acc += a.chunk_tot[(int64_t)c * n + t];
+        coef[t] = acc;
+    }
+    __syncthreads();
+    const PRIPPS_CONST DevStruct &sd = as_const(a.structs)[sa.x];
+    const int strip = sa.y, ql = threadIdx.x;
+    const int first = sd.strip_first[strip], end = sd.strip_first[strip + 1];
+    const int m = first + ql;
+    double v[kProfiles] = {0, 0, 0, 0, 0, 0};
+    if (m < end) {
+        const double qc = sd.strip_geo[2 * strip], H = sd.strip_geo[2 * strip + 1];
+        const double x = H > 0.0 ? (sd.q[m] - qc) / H : 0.0;
+        double s[kChebSets];
+        for (int set = 0; set < kChebSets; ++set) {
+            const double *c = coef + set * a.RS + sa.z * a.P;
+            double b1 = 0.0, b2 = 0.0;
+            for (int p = a.P - 1; p >= 1; --p) {
+                const double b0 = __builtin_fma(2.0 * x, b1, c[p] - b2);
+                b2 = b1;
+                b1 = b0;
+            }
+            s[set] = __builtin_fma(x, b1, c[0] - b2);
+        }
+        const int nq = sd.nq, ik = sa.w, jk = sb.x;
+        const double ak = sd.ffa[ik * nq + m], al = sd.ffa[jk * nq + m];
+        const double ek = sd.ffe ? sd.ffe[ik * nq + m] : 0.0;
+        const double el = sd.ffe ? sd.ffe[jk * nq + m] : 0.0;
+        const double wq = sd.ffw ? sd.ffw[m] : 0.0;
+        // sets: S = sum s, Si = sum w_i s, Sw = sum w_j s, Siw = sum w_i w_j s
+        v[0] = 2.0 * ak * al * s[0];
+        v[1] = 2.0 * ek * el * s[0];
+        v[2] = 2.0 * wq * wq * s[3];
+        v[3] = (ak * el + al * ek) * s[0];
+        v[4] = wq * (al * s[1]) + wq * (ak * s[2]);
+        v[5] = wq * (el * s[1]) + wq * (ek * s[2]);
+    }
+    double *o = a.sb_out + (int64_t)blockIdx.x * (kProfiles * kChebQ) + ql;
+#pragma unroll
+    for (int pr = 0; pr < kProfiles; ++pr) o[pr * kChebQ] = v[pr];
+}
+
+// Stage 3: block t adds the combo contributions of its strip in order,
+// divides by q and adds the diagonal (q = 0: closed form).
+__global__ void __launch_bounds__(kChebQ) cheb_strip_kernel(ChebReduceArgs a) {
+    const int4 st = a.st[blockIdx.x];
+    const PRIPPS_CONST DevStruct &sd = as_const(a.structs)[st.x];
+    const int strip = st.y, ql = threadIdx.x;
+    const int m = sd.strip_first[strip] + ql;
+    if (m >= sd.strip_first[strip + 1]) return;
+    double acc[kProfiles] = {0, 0, 0, 0, 0, 0};
+    for (int b = st.z; b < st.w; ++b) {
+        const double *p = a.sb_out + (int64_t)b * (kProfiles * kChebQ) + ql;
+#pragma unroll
+        for (int pr = 0; pr < kProfiles; ++pr) acc[pr] += p[pr * kChebQ];
+    }
+    double diag[kProfiles], zero[kProfiles];
+    diagonal_and_zero(sd, m, diag, zero);
+    const double q = sd.q[m];
+#pragma unroll
+    for (int pr = 0; pr < kProfiles; ++pr)
+        sd.profiles[pr * sd.nq + m] = q > 0.0 ? acc[pr] / q + diag[pr] : zero[pr];
+}
+
+hipError_t launch_cheb_reduce(const ChebReduceArgs &a, hipStream_t st) {
+    if (a.n_chunks > 0) {
+        hipLaunchKernelGGL(cheb_chunk_kernel, dim3(a.n_chunks), dim3(256), 0, st, a);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (a.n_sb > 0) {
+        hipLaunchKernelGGL(cheb_combo_kernel, dim3(a.n_sb), dim3(kChebQ), 0, st, a);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (a.n_st > 0) {
+        hipLaunchKernelGGL(cheb_strip_kernel, dim3(a.n_st), dim3(kChebQ), 0, st, a);
+        return hipGetLastError();
+    }
+    return hipSuccess;
 }
 
 // Orders the kernel is built for; the host rounds a structure's need up.

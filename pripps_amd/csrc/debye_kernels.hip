@@ -790,38 +790,6 @@ __global__ void __launch_bounds__(256) debye_dual_kernel(TiledArgs args) {
 // finalize: sum item partials per (structure, strip) in a fixed order, divide
 // by q, add the diagonal; q = 0 takes the closed form (sum f_X)(sum f_Y).
 // ---------------------------------------------------------------------------
-__device__ void diagonal_and_zero(const PRIPPS_CONST DevStruct &sd, int m, double (&diag)[kProfiles],
-                                  double (&zero)[kProfiles]) {
-    double aa = 0, ee = 0, ae = 0, sa = 0, se = 0, aw = 0, ew = 0, sw = 0;
-    for (int k = 0; k < sd.K; ++k) {
-        const double n = sd.kind_n[k];
-        const double a = sd.ffa[k * sd.nq + m];
-        const double e = sd.ffe ? sd.ffe[k * sd.nq + m] : 0.0;
-        const double wk = sd.kind_w[k];
-        aa += n * a * a;
-        ee += n * e * e;
-        ae += n * a * e;
-        sa += n * a;
-        se += n * e;
-        aw += a * wk;
-        ew += e * wk;
-        sw += wk;
-    }
-    const double W = sd.ffw ? sd.ffw[m] : 0.0;
-    diag[0] = aa;
-    diag[1] = ee;
-    diag[2] = W * W * sd.sum_w2;
-    diag[3] = ae;
-    diag[4] = W * aw;
-    diag[5] = W * ew;
-    const double sh = W * sw;
-    zero[0] = sa * sa;
-    zero[1] = se * se;
-    zero[2] = sh * sh;
-    zero[3] = sa * se;
-    zero[4] = sa * sh;
-    zero[5] = se * sh;
-}
 
 // Stage 1: block (strip g, chunk c) sums the partials of items
 // csr[g][c*kFinChunk .. (c+1)*kFinChunk) into stage[g][c][6][Q].  Thread t

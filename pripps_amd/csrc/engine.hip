@@ -10,11 +10,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <climits>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <atomic>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -151,7 +154,8 @@ struct LaunchRecord {
     int64_t n_items = 0;
     int *counter = nullptr;
     TiledArgs targs{};
-    FinalizeArgs fargs{};
+    FinalizeArgs fargs{};     // recurrence kernels
+    ChebReduceArgs cargs{};   // Chebyshev kernel
 };
 
 // fill counter, pair kernel (bracketed by ev_a/ev_b when given), reduction
@@ -162,7 +166,8 @@ hipError_t run_launch(const LaunchRecord &r, hipStream_t st, hipEvent_t ev_a, hi
         err = r.Q == kChebQ ? launch_cheb(r.P, r.targs, r.blocks, st)
                             : launch_tiled(r.fp64, r.Q, r.uniform, r.targs, r.blocks, st);
     if (err == hipSuccess && ev_b) err = hipEventRecord(ev_b, st);
-    if (err == hipSuccess && r.n_strips > 0) err = launch_finalize(r.fargs, r.n_strips, st);
+    if (err == hipSuccess && r.Q == kChebQ) err = launch_cheb_reduce(r.cargs, st);
+    else if (err == hipSuccess && r.n_strips > 0) err = launch_finalize(r.fargs, r.n_strips, st);
     return err;
 }
 }  // namespace pripps
@@ -554,13 +559,23 @@ namespace {
 
 struct StructPlan {
     int Q = 64, span = 256, block = 64, variant = 3;
-    // Chebyshev plans (Q == kChebQ): expansion order and strips
+    bool span_set = false;  // span from the caller's tuning or the tuning table
+    // Chebyshev plans (Q == kChebQ): expansion order, strips, and the groups
+    // of one or two consecutive strips that share a half-width (one work
+    // item covers a group)
     int P = 0;
     std::vector<int32_t> strip_first;  // [n_strips + 1]
     std::vector<double> strip_geo;     // [n_strips][2]: centre, half-width
+    std::vector<int32_t> group_first;  // [n_groups + 1]: first strip of each group
     int n_strips(int nq) const {
         return Q == kChebQ ? (int)strip_first.size() - 1 : (nq + Q - 1) / Q;
     }
+    // work units per (tile, j-span): strip groups, or strips
+    int n_units(int nq) const {
+        return Q == kChebQ ? (int)group_first.size() - 1 : n_strips(nq);
+    }
+    int unit_first(int u) const { return Q == kChebQ ? group_first[u] : u; }
+    int unit_strips(int u) const { return Q == kChebQ ? group_first[u + 1] - group_first[u] : 1; }
 };
 
 // Q: q-points per lane strip of the recurrence kernels (32, 64, 128), or
@@ -571,13 +586,17 @@ int valid_q(int precision, bool uniform, int Q) {
 }
 
 // Strips of a Chebyshev plan: runs of up to kChebQ consecutive q-points
-// whose half-width H keeps beta = H r_max within what order P admits.  Each
-// built order is tried with greedy strips and the one with the fewest
-// instructions per pair over the grid wins (about 60 + 4.5 P per pair and
-// strip).  The expansion's error is about uniform in absolute terms across a
-// strip, while the profiles divide the pair sum by q, so a strip also keeps
+// whose half-width H keeps beta = H r_max within what order P admits.  The
+// expansion's error is about uniform in absolute terms across a strip, while
+// the profiles divide the pair sum by q, so a strip also keeps
 // H <= kChebLowQ * (its smallest non-zero q): that bounds the amplification
 // of the error at the strip's low end (q = 0 itself takes the closed form).
+// Consecutive strips then pair up into groups that share the larger H (the
+// pair's Bessel weights depend on beta alone, so one recurrence serves both)
+// where that H still meets both strips' low-q bound.  Each built order is
+// tried with greedy strips and the one with the fewest instructions per pair
+// over the grid wins: about 32 + 4 P per group (geometry, recurrence, the
+// scaled weights) and 22 + P per strip (its sine and cosine, its P sums).
 constexpr double kChebLowQ = 128.0;
 
 void cheb_strips(const Prepared &p, StructPlan &sp) {
@@ -595,7 +614,7 @@ void cheb_strips(const Prepared &p, StructPlan &sp) {
     for (int k = 0; k < 7; ++k) {
         const double hmax = limit[k] / rplan;
         std::vector<int32_t> first{0};
-        std::vector<double> geo;
+        std::vector<double> geo, nzs;
         int m0 = 0;
         while (m0 < nq) {
             double lo = p.q[m0], hi = p.q[m0];
@@ -614,14 +633,33 @@ void cheb_strips(const Prepared &p, StructPlan &sp) {
             first.push_back(m1);
             geo.push_back(0.5 * (lo + hi));
             geo.push_back(0.5 * (hi - lo));
+            nzs.push_back(nz);
             m0 = m1;
         }
-        const double cost = (double)(first.size() - 1) * (60.0 + 4.5 * orders[k]);
+        const int ns = (int)first.size() - 1;
+        std::vector<int32_t> groups{0};
+        for (int st = 0; st < ns;) {
+            if (st + 1 < ns) {
+                const double hg = std::max(geo[2 * st + 1], geo[2 * st + 3]);
+                const double nzg = std::min(nzs[st], nzs[st + 1]);
+                if (nzg == HUGE_VAL || hg <= kChebLowQ * nzg) {
+                    geo[2 * st + 1] = geo[2 * st + 3] = hg;
+                    st += 2;
+                    groups.push_back(st);
+                    continue;
+                }
+            }
+            st += 1;
+            groups.push_back(st);
+        }
+        const double cost = (double)(groups.size() - 1) * (32.0 + 4.0 * orders[k]) +
+                            (double)ns * (22.0 + orders[k]);
         if (k == 0 || cost < best_cost) {
             best_cost = cost;
             sp.P = orders[k];
             sp.strip_first = std::move(first);
             sp.strip_geo = std::move(geo);
+            sp.group_first = std::move(groups);
         }
     }
 }
@@ -661,8 +699,8 @@ int default_q(int precision, bool uniform, int nq, int n_atoms) {
 // items per wave slot of the GPU (64..8192): on the 10,000-atom, 500-q
 // globule span 192 (8,568 items) measured 0.520 ms against 0.593 ms at 256
 // (6,464) and 0.525 at 128 (12,768, more partial slots to reduce).
-int default_span(const Prepared &p, int Q, int n_cu, int n_strips = 0) {
-    const double strips = n_strips > 0 ? (double)n_strips : (double)((p.nq + Q - 1) / Q);
+int default_span(const Prepared &p, int Q, int n_cu, int n_units = 0) {
+    const double strips = n_units > 0 ? (double)n_units : (double)((p.nq + Q - 1) / Q);
     const double work = (double)p.tiles.size() * 0.5 * p.n * strips;
     if (Q == kChebQ) {
         const double items = 3.0 * 12.0 * std::max(1, n_cu);
@@ -681,6 +719,12 @@ int default_span(const Prepared &p, int Q, int n_cu, int n_strips = 0) {
 // coherently within a block: on a grid hydration shell at its first minimum
 // the error was 1.1e-5 at 256, 4.5e-6 at 64 and 1.8e-6 at 8.
 constexpr int kDefaultBlock = 64;
+// Chebyshev plans sum each coefficient over the chunk in FP32 and evaluate
+// the sums at every q-point of the strip, so their rounding counts about
+// sqrt(P) times more than a sum per q-point: a shorter chunk keeps the error
+// where the recurrence kernels have it, and the carry per chunk is a few
+// lane swaps and FP64 adds.
+constexpr int kChebBlock = 32;
 
 int shifted_span(int base, int shift) {
     double s = std::ldexp((double)base, shift);
@@ -710,6 +754,15 @@ struct LaunchGroup {
     int Q = 64, variant = 3, P = 0;
     std::vector<int> members;
     std::vector<Item> items;
+    int64_t n_slots = 0;  // partial slots (recurrence kernels): one per item
+    // Chebyshev plans: the queue of units, and the reduction of their records
+    std::vector<Unit> units;
+    int RS = 0;
+    int64_t n_records = 0;
+    std::vector<int32_t> chunk_recs, chunk_first;
+    std::vector<int4> sb_a, sb_b, st;
+    size_t o_units = 0, o_chunk_recs = 0, o_chunk_first = 0, o_sb_a = 0, o_sb_b = 0, o_st = 0;
+    size_t o_records = 0, o_chunk_tot = 0, o_sb_out = 0;
     std::vector<int32_t> csr_ptr, csr_items, strip_struct, strip_index, stage_ptr;
     size_t o_items = 0, o_csr_ptr = 0, o_csr_items = 0, o_ss = 0, o_si = 0, o_sp = 0;
     size_t o_counter = 0, o_partials = 0, o_stage = 0;
@@ -752,15 +805,173 @@ static StructPlan resolve_plan(pripps_engine *e, const pripps_tuning *tune, int 
     if (!Q) Q = default_q(precision, p.uniform, p.nq, p.n);
     sp.Q = Q;
     if (Q == kChebQ) cheb_strips(p, sp);
-    const int base = default_span(p, Q, e->n_cu, Q == kChebQ ? sp.n_strips(p.nq) : 0);
+    const int base = default_span(p, Q, e->n_cu, Q == kChebQ ? sp.n_units(p.nq) : 0);
     if (expl && tune->span > 0) sp.span = std::max(64, tune->span);
     else if (have) sp.span = shifted_span(base, te.span_shift);
     else sp.span = base;
-    const int block =
-        expl && tune->block > 0 ? tune->block : (have && te.block > 0 ? te.block : kDefaultBlock);
+    sp.span_set = (expl && tune->span > 0) || have;
+    const int block = expl && tune->block > 0 ? tune->block
+                      : have && te.block > 0  ? te.block
+                      : Q == kChebQ           ? kChebBlock
+                                              : kDefaultBlock;
     sp.block = std::max(8, std::min(block, 4096));
     sp.variant = valid_variant(precision, p.uniform, Q, expl ? tune->variant : (have ? te.variant : 3));
     return sp;
+}
+
+// Units of a Chebyshev launch.  A structure's work for one strip group is the
+// sequence of its tiles' j ranges in order, each j costing one pass (two for
+// hydrated j, whose chunks run the w_j pass too); it is cut into units of
+// equal cost -- contiguous runs, so a unit holds one or a few items -- and
+// each unit sums into its own partial slots, which the reduction adds in a
+// fixed order.  A structure that can fill the GPU gets one unit per wave
+// slot of the launch, so that the waves finish together; smaller ones get
+// units of about kChebUnitMin j atoms; a span from the caller's tuning sets
+// the unit length instead (in j atoms of a two-strip group).  The cut
+// depends on the structure alone, so its profiles are bitwise the same alone
+// or in any batch.  The queue runs the units costliest first (a stable sort).
+constexpr int64_t kChebUnitMin = 48;
+
+static void plan_cheb_group(LaunchGroup &g, const std::vector<Prepared> &P,
+                            const std::vector<StructPlan> &sp, int n_waves) {
+    struct Pending {
+        double cost;
+        Unit u;
+        int32_t s, strip;
+    };
+    std::vector<Pending> pend;
+    g.items.clear();
+    for (int s : g.members) {
+        const Prepared &p = P[s];
+        const StructPlan &q = sp[s];
+        const int ng = q.n_units(p.nq);
+        if (ng == 0 || p.tiles.empty()) continue;
+        // hydrated atoms before j, from the class ranges
+        auto wet_before = [&](int32_t j) {
+            int64_t w = 0;
+            for (int c = 0; c < p.n_classes; ++c)
+                if (p.cls_wet[c]) {
+                    const int32_t lo = p.cls_start[c], hi = std::min(p.cls_start[c + 1], j);
+                    if (hi > lo) w += hi - lo;
+                }
+            return w;
+        };
+        auto cost = [&](int32_t a, int32_t b) { return (int64_t)(b - a) + wet_before(b) - wet_before(a); };
+        int64_t W = 0;
+        for (const int4 &t : p.tiles) W += cost(t.x, p.n);
+        if (W <= 0) continue;
+        std::vector<double> f(ng);
+        double fs = 0.0;
+        for (int u = 0; u < ng; ++u) {
+            f[u] = 32.0 + 4.0 * q.P + q.unit_strips(u) * (22.0 + q.P);
+            fs += f[u];
+        }
+        const double fref = 32.0 + 4.0 * q.P + 2.0 * (22.0 + q.P);
+        const double wref = (double)W * fs / fref;  // cost in j atoms of a two-strip group
+        const int64_t us =
+            q.span_set ? std::max<int64_t>(1, (int64_t)std::ceil(wref / q.span))
+                       : std::max<int64_t>(1, std::min<int64_t>(n_waves, (int64_t)(wref / (double)kChebUnitMin)));
+        for (int u = 0; u < ng; ++u) {
+            const int nk = q.unit_strips(u);
+            const int strip = q.unit_first(u);
+            int64_t ug = std::max<int64_t>(1, std::llround((double)us * f[u] / fs));
+            ug = std::min(ug, W);
+            size_t t = 0;
+            int32_t j = p.tiles[0].x;
+            int64_t done = 0, prev = 0;
+            for (int64_t k = 0; k < ug && t < p.tiles.size(); ++k) {
+                const int64_t target = k + 1 == ug ? W : W * (k + 1) / ug;
+                const int32_t item0 = (int32_t)g.items.size();
+                while (done < target && t < p.tiles.size()) {
+                    const int4 &tl = p.tiles[t];
+                    const int64_t rest = cost(j, p.n);
+                    int32_t j1 = p.n;
+                    if (done + rest > target) {
+                        // the first j1 with cost(j, j1) >= target - done
+                        int32_t lo = j + 1, hi = p.n;
+                        while (lo < hi) {
+                            const int32_t mid = lo + (hi - lo) / 2;
+                            if (cost(j, mid) >= target - done) hi = mid;
+                            else lo = mid + 1;
+                        }
+                        j1 = lo;
+                    }
+                    g.items.push_back(Item{s, tl.x, tl.y, tl.z, j, j1, strip, nk - 1});
+                    done += cost(j, j1);
+                    if (j1 == p.n) {
+                        if (++t < p.tiles.size()) j = p.tiles[t].x;
+                    } else {
+                        j = j1;
+                    }
+                }
+                const int32_t item1 = (int32_t)g.items.size();
+                if (item1 > item0)
+                    pend.push_back(Pending{(double)(done - prev) * f[u], Unit{item0, item1, 0, nk}, s, strip});
+                prev = done;
+            }
+        }
+    }
+    std::vector<int32_t> order(pend.size());
+    for (size_t x = 0; x < order.size(); ++x) order[x] = (int32_t)x;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t b) { return pend[a].cost > pend[b].cost; });
+    g.units.resize(pend.size());
+    for (size_t x = 0; x < order.size(); ++x) g.units[x] = pend[order[x]].u;
+    // Records: one per class run of each item, unit by unit in queue order
+    // (the order in which the kernel numbers them from the unit's rec0).
+    // Combos (structure, group's first strip, i kind, j kind) list their
+    // records in that order; a structure's lists do not depend on the batch.
+    g.RS = (2 * g.P + 63) / 64 * 64;
+    std::map<std::array<int32_t, 4>, std::vector<int32_t>> combos;
+    int64_t rec = 0;
+    for (Unit &u : g.units) {
+        u.rec0 = (int32_t)rec;
+        for (int32_t i = u.item0; i < u.item1; ++i) {
+            const Item &it = g.items[i];
+            const Prepared &p = P[it.s];
+            for (int c = 0; c < p.n_classes; ++c) {
+                const int32_t lo = std::max(it.j0, p.cls_start[c]);
+                const int32_t hi = std::min(it.j1, p.cls_start[c + 1]);
+                if (lo >= hi) continue;
+                combos[{it.s, it.strip, it.ikind, (int32_t)p.cls_kind[c]}].push_back((int32_t)rec++);
+            }
+        }
+    }
+    g.n_records = rec;
+    // stage 1 chunks, combo by combo in key order; stage 2 blocks per strip
+    // and combo; stage 3 blocks per strip (every strip, with or without work)
+    g.chunk_recs.clear();
+    g.chunk_first.assign(1, 0);
+    std::map<std::array<int32_t, 4>, std::array<int32_t, 2>> combo_chunks;
+    for (const auto &kv : combos) {
+        const int32_t c0 = (int32_t)g.chunk_first.size() - 1;
+        const auto &lst = kv.second;
+        for (size_t x = 0; x < lst.size(); x += kFinChunk) {
+            const size_t e = std::min(lst.size(), x + (size_t)kFinChunk);
+            g.chunk_recs.insert(g.chunk_recs.end(), lst.begin() + x, lst.begin() + e);
+            g.chunk_first.push_back((int32_t)g.chunk_recs.size());
+        }
+        combo_chunks[kv.first] = {c0, (int32_t)g.chunk_first.size() - 1};
+    }
+    g.sb_a.clear();
+    g.sb_b.clear();
+    g.st.clear();
+    for (int s : g.members) {
+        const StructPlan &q = sp[s];
+        const int ng = q.n_units(P[s].nq);
+        for (int u = 0; u < ng; ++u) {
+            const int gf = q.unit_first(u);
+            for (int k = 0; k < q.unit_strips(u); ++k) {
+                const int32_t b0 = (int32_t)g.sb_a.size();
+                for (auto it = combo_chunks.lower_bound({s, gf, INT32_MIN, INT32_MIN});
+                     it != combo_chunks.end() && it->first[0] == s && it->first[1] == gf; ++it) {
+                    g.sb_a.push_back(int4{s, gf + k, k, it->first[2]});
+                    g.sb_b.push_back(int4{it->first[3], it->second[0], it->second[1], 0});
+                }
+                g.st.push_back(int4{s, gf + k, b0, (int32_t)g.sb_a.size()});
+            }
+        }
+    }
 }
 
 static void plan_group(LaunchGroup &g, const std::vector<Prepared> &P,
@@ -773,26 +984,27 @@ static void plan_group(LaunchGroup &g, const std::vector<Prepared> &P,
     auto each_item = [&](auto &&emit) {
         for (int s : g.members) {
             const Prepared &p = P[s];
-            const int nst = sp[s].n_strips(p.nq);
+            const int nu = sp[s].n_units(p.nq);
             const int span = sp[s].span;
             for (const int4 &t : p.tiles)
                 for (int32_t j0 = t.x; j0 < p.n; j0 += span)
-                    emit(s, t, j0, std::min<int32_t>(p.n, j0 + span), nst);
+                    emit(s, t, j0, std::min<int32_t>(p.n, j0 + span), nu);
         }
     };
     int maxlen = 0;
     for (int s : g.members) maxlen = std::max(maxlen, sp[s].span);
     std::vector<int64_t> pos(maxlen + 2, 0);
-    each_item([&](int, const int4 &, int32_t j0, int32_t j1, int nst) {
-        pos[maxlen - (j1 - j0) + 1] += nst;
+    each_item([&](int, const int4 &, int32_t j0, int32_t j1, int nu) {
+        pos[maxlen - (j1 - j0) + 1] += nu;
     });
     for (int x = 0; x <= maxlen; ++x) pos[x + 1] += pos[x];
     g.items.resize((size_t)pos[maxlen + 1]);
     Item *items = g.items.data();
-    each_item([&](int s, const int4 &t, int32_t j0, int32_t j1, int nst) {
+    each_item([&](int s, const int4 &t, int32_t j0, int32_t j1, int nu) {
         int64_t &w = pos[maxlen - (j1 - j0)];
-        for (int st = 0; st < nst; ++st) items[w++] = Item{s, t.x, t.y, t.z, j0, j1, st, 0};
+        for (int u = 0; u < nu; ++u) items[w++] = Item{s, t.x, t.y, t.z, j0, j1, u, 0};
     });
+    g.n_slots = (int64_t)g.items.size();  // item w sums into slot w
     // CSR of items per (member, strip), in queue order
     const int nm = (int)g.members.size();
     std::vector<int64_t> base(nm + 1, 0);
@@ -852,8 +1064,15 @@ static void plan_build(pripps_engine *e, const std::vector<Prepared> &P, int pre
             g->members.push_back(s);
         }
         for (auto &g : B.groups) {
-            plan_group(g, P, B.sp);
-            B.n_items += (int64_t)g.items.size();
+            if (g.Q == kChebQ) {
+                int bpc = 1;
+                if (cheb_occupancy(g.P, &bpc) != hipSuccess) bpc = 1;
+                plan_cheb_group(g, P, B.sp, e->n_cu * 4 * std::max(1, bpc));
+                B.n_items += (int64_t)g.units.size();
+            } else {
+                plan_group(g, P, B.sp);
+                B.n_items += (int64_t)g.items.size();
+            }
         }
     }
     Layout L;
@@ -877,6 +1096,12 @@ static void plan_build(pripps_engine *e, const std::vector<Prepared> &P, int pre
     B.o_structs = L.take(ns * sizeof(DevStruct));
     for (auto &g : B.groups) {
         g.o_items = L.take(g.items.size() * sizeof(Item) + 16);
+        g.o_units = L.take(g.units.size() * sizeof(Unit) + 16);
+        g.o_chunk_recs = L.take(g.chunk_recs.size() * 4 + 4);
+        g.o_chunk_first = L.take(g.chunk_first.size() * 4 + 4);
+        g.o_sb_a = L.take(g.sb_a.size() * sizeof(int4) + 16);
+        g.o_sb_b = L.take(g.sb_b.size() * sizeof(int4) + 16);
+        g.o_st = L.take(g.st.size() * sizeof(int4) + 16);
         g.o_csr_ptr = L.take(g.csr_ptr.size() * 4 + 4);
         g.o_csr_items = L.take(g.csr_items.size() * 4 + 4);
         g.o_ss = L.take(g.strip_struct.size() * 4 + 4);
@@ -891,7 +1116,13 @@ static void plan_build(pripps_engine *e, const std::vector<Prepared> &P, int pre
             B.o_pp[s] = L.take(std::max<size_t>(1, P[s].tiles.size() * (size_t)P[s].nq * kProfiles) * 8);
     }
     for (auto &g : B.groups) {
-        g.o_partials = L.take(std::max<size_t>(1, g.items.size()) * kProfiles * g.Q * 8);
+        g.o_partials = L.take(std::max<int64_t>(1, g.n_slots) * kProfiles * g.Q * 8);
+        if (g.Q == kChebQ) {
+            const int64_t n_chunks = (int64_t)g.chunk_first.size() - 1;
+            g.o_records = L.take(std::max<int64_t>(1, g.n_records) * kChebSets * g.RS * 8);
+            g.o_chunk_tot = L.take(std::max<int64_t>(1, n_chunks) * kChebSets * g.RS * 8);
+            g.o_sb_out = L.take(std::max<size_t>(1, g.sb_a.size()) * kProfiles * kChebQ * 8);
+        }
         const size_t nch = g.stage_ptr.empty() ? 0 : (size_t)g.stage_ptr.back();
         g.o_stage = L.take(std::max<size_t>(1, nch) * kProfiles * g.Q * 8);
     }
@@ -963,6 +1194,12 @@ static void stage_build(const BuildPlan &B, const std::vector<Prepared> &P, char
     };
     for (auto &g : B.groups) {
         put(g.o_items, g.items.data(), g.items.size() * sizeof(Item));
+        put(g.o_units, g.units.data(), g.units.size() * sizeof(Unit));
+        put(g.o_chunk_recs, g.chunk_recs.data(), g.chunk_recs.size() * 4);
+        put(g.o_chunk_first, g.chunk_first.data(), g.chunk_first.size() * 4);
+        put(g.o_sb_a, g.sb_a.data(), g.sb_a.size() * sizeof(int4));
+        put(g.o_sb_b, g.sb_b.data(), g.sb_b.size() * sizeof(int4));
+        put(g.o_st, g.st.data(), g.st.size() * sizeof(int4));
         put(g.o_csr_ptr, g.csr_ptr.data(), g.csr_ptr.size() * 4);
         put(g.o_csr_items, g.csr_items.data(), g.csr_items.size() * 4);
         put(g.o_ss, g.strip_struct.data(), g.strip_struct.size() * 4);
@@ -981,7 +1218,8 @@ static int make_launches(pripps_engine *e, const BuildPlan &B, char *dbase,
         r.P = g.P;
         r.uniform = g.uniform;
         r.counter = (int *)(dbase + g.o_counter);
-        r.n_items = (int64_t)g.items.size();
+        const bool cheb = g.Q == kChebQ;
+        r.n_items = cheb ? (int64_t)g.units.size() : (int64_t)g.items.size();
         r.n_strips = (int)g.strip_struct.size();
         TiledArgs &a = r.targs;
         a.structs = (const DevStruct *)(dbase + B.o_structs);
@@ -990,6 +1228,25 @@ static int make_launches(pripps_engine *e, const BuildPlan &B, char *dbase,
         a.counter = r.counter;
         a.partials = (double *)(dbase + g.o_partials);
         a.variant = g.variant;
+        a.units = cheb ? (const Unit *)(dbase + g.o_units) : nullptr;
+        if (cheb) {
+            a.partials = (double *)(dbase + g.o_records);
+            ChebReduceArgs &c = r.cargs;
+            c.structs = a.structs;
+            c.records = a.partials;
+            c.RS = g.RS;
+            c.P = g.P;
+            c.chunk_recs = (const int32_t *)(dbase + g.o_chunk_recs);
+            c.chunk_first = (const int32_t *)(dbase + g.o_chunk_first);
+            c.n_chunks = (int)g.chunk_first.size() - 1;
+            c.chunk_tot = (double *)(dbase + g.o_chunk_tot);
+            c.sb_a = (const int4 *)(dbase + g.o_sb_a);
+            c.sb_b = (const int4 *)(dbase + g.o_sb_b);
+            c.n_sb = (int)g.sb_a.size();
+            c.sb_out = (double *)(dbase + g.o_sb_out);
+            c.st = (const int4 *)(dbase + g.o_st);
+            c.n_st = (int)g.st.size();
+        }
         int bpc = 1;
         if (g.Q == kChebQ) HIPCHK(cheb_occupancy(g.P, &bpc));
         else HIPCHK(tiled_occupancy(B.fp64, g.Q, g.uniform, g.variant, &bpc));
@@ -1005,8 +1262,8 @@ static int make_launches(pripps_engine *e, const BuildPlan &B, char *dbase,
         f.strip_index = (const int32_t *)(dbase + g.o_si);
         f.stage_ptr = (const int32_t *)(dbase + g.o_sp);
         f.stage = (double *)(dbase + g.o_stage);
-        f.n_strips = (int)(g.stage_ptr.size() - 1);
-        f.total_chunks = g.stage_ptr.back();
+        f.n_strips = g.stage_ptr.empty() ? 0 : (int)(g.stage_ptr.size() - 1);
+        f.total_chunks = g.stage_ptr.empty() ? 0 : g.stage_ptr.back();
         f.Q = g.Q;
         launches.push_back(r);
     }

@@ -18,6 +18,10 @@ constexpr int kFinChunk = 32; // item partials summed per block in the first red
 // that kernel are runs of up to kChebQ consecutive q-points with their own
 // centre and half-width, on any grid.
 constexpr int kChebQ = 256;
+// FP64 coefficient sums per class run of a Chebyshev item: S = sum s,
+// Si = sum w_i s (pass 0), Sw = sum w_j s, Siw = sum w_i w_j s (pass 1,
+// hydrated j)
+constexpr int kChebSets = 4;
 
 // One atom of the sorted SoA-in-AoS stream: centred coordinates and the
 // hydration weight, 16 bytes (FP32) or 32 bytes (FP64) per atom so a wave
@@ -68,14 +72,47 @@ struct Item {
     int32_t pad;
 };
 
+// Chebyshev plans: one wave's share of a structure's work for one strip
+// group -- a contiguous run of items, cut so that the units of a structure
+// cost about the same -- summed into one partial slot per strip.
+struct Unit {
+    int32_t item0, item1;  // items [item0, item1)
+    int32_t rec0;          // record of its first class run (one per run, in order)
+    int32_t nk;            // strips in the group (1 or 2)
+};
+
+// Reduction of a Chebyshev launch.  A record holds the FP64 coefficient sums
+// of one class run: [kChebSets][RS] (set, then coefficient k P + p of strip k
+// of the group; RS = 64 or 128).  A combo is (structure, strip group, i kind,
+// j kind); its records are summed in chunks, the chunk totals of a combo are
+// summed and evaluated at the q-points of each strip of the group (stage 2,
+// one block per (strip, combo)), and stage 3 adds a strip's combos, divides
+// by q and adds the diagonal.
+struct ChebReduceArgs {
+    const DevStruct *structs;
+    const double *records;       // [records][kChebSets][RS]
+    int RS, P;
+    const int32_t *chunk_recs;   // record ids, combo by combo, in queue order
+    const int32_t *chunk_first;  // [n_chunks + 1] into chunk_recs
+    int n_chunks;
+    double *chunk_tot;           // [n_chunks][kChebSets][RS]
+    const int4 *sb_a;            // stage-2 block: (structure, strip, k in group, i kind)
+    const int4 *sb_b;            // (j kind, first chunk, end chunk, 0)
+    int n_sb;
+    double *sb_out;              // [n_sb][6][kChebQ]
+    const int4 *st;              // stage-3 block: (structure, strip, first sb, end sb)
+    int n_st;
+};
+
 // Kernel argument blocks.
 struct TiledArgs {
     const DevStruct *structs;
     const Item *items;
-    int64_t n_items;
+    int64_t n_items;   // queue length: items, or units (Chebyshev plans)
     int *counter;      // work-queue head, zeroed before launch
-    double *partials;  // [n_items][6][Q]
+    double *partials;  // [slots][6][Q]; a slot per item, or per unit and strip
     int variant;       // FP32 uniform inner-loop variant (see debye_tiled_kernel)
+    const Unit *units; // Chebyshev plans, else nullptr
 };
 
 struct FinalizeArgs {
@@ -116,6 +153,7 @@ hipError_t launch_finalize(const FinalizeArgs &a, int n_strips_total, hipStream_
 // Chebyshev pair kernel (debye_cheb.hip), P = expansion order (a built order)
 hipError_t launch_cheb(int P, const TiledArgs &a, int blocks, hipStream_t st);
 hipError_t cheb_occupancy(int P, int *blocks_per_cu);
+hipError_t launch_cheb_reduce(const ChebReduceArgs &a, hipStream_t st);
 // smallest built order whose truncation is below 1e-9 of a pair's amplitude
 // at beta = H r up to beta_max (0: none)
 int cheb_order_for(double beta_max);

@@ -56,7 +56,7 @@ __device__ __forceinline__ Item load_item(const Item *items, int64_t w) {
     it.j0 = p->j0;
     it.j1 = p->j1;
     it.strip = p->strip;
-    it.pad = 0;
+    it.pad = p->pad;
     return it;
 }
 
@@ -131,6 +131,41 @@ __device__ __forceinline__ void sqrt_and_rsqrt(float r2, float &r, float &rinv) 
     r = __builtin_fmaf(d, 0.5f * y, r0);
     const float e = __builtin_fmaf(-r, y, 1.0f);
     rinv = __builtin_fmaf(y, e, y);
+}
+
+// The i = j terms of the six profiles at q-point m (sinc(0) = 1), and the
+// profiles at q = 0 in closed form, (sum f_X)(sum f_Y).
+__device__ inline void diagonal_and_zero(const PRIPPS_CONST DevStruct &sd, int m, double (&diag)[kProfiles],
+                                  double (&zero)[kProfiles]) {
+    double aa = 0, ee = 0, ae = 0, sa = 0, se = 0, aw = 0, ew = 0, sw = 0;
+    for (int k = 0; k < sd.K; ++k) {
+        const double n = sd.kind_n[k];
+        const double a = sd.ffa[k * sd.nq + m];
+        const double e = sd.ffe ? sd.ffe[k * sd.nq + m] : 0.0;
+        const double wk = sd.kind_w[k];
+        aa += n * a * a;
+        ee += n * e * e;
+        ae += n * a * e;
+        sa += n * a;
+        se += n * e;
+        aw += a * wk;
+        ew += e * wk;
+        sw += wk;
+    }
+    const double W = sd.ffw ? sd.ffw[m] : 0.0;
+    diag[0] = aa;
+    diag[1] = ee;
+    diag[2] = W * W * sd.sum_w2;
+    diag[3] = ae;
+    diag[4] = W * aw;
+    diag[5] = W * ew;
+    const double sh = W * sw;
+    zero[0] = sa * sa;
+    zero[1] = se * se;
+    zero[2] = sh * sh;
+    zero[3] = sa * se;
+    zero[4] = sa * sh;
+    zero[5] = se * sh;
 }
 
 }  // namespace pripps

@@ -113,11 +113,15 @@ def test_server_alone_equals_in_mix_and_admission(engine, foxs_engine):
         assert np.array_equal(m.profiles(), engine.build(inp, "fp32").profiles())
         total = m.intensity(1.05, 0.5)[0]
         assert np.all(np.isfinite(total))
-    # admission: a budget smaller than one request fails that request only
-    with Server(engine, "fp32", memory_budget=32_000_000) as srv:
-        small = srv.submit(inps[0])
+    # admission: a budget smaller than one request fails that request only.
+    # A 500-atom, 100-q request needs well under 1 MB of device memory; the
+    # 20,000-atom, 800-q one over 10 MB (its coefficient records alone).
+    snames, spos, sw = globule(500, radius_for(500), 76)
+    tiny = foxs_engine.prepare_workload(Workload("small", snames, spos, linear_q(0, 0.5, 100), sw))
+    with Server(engine, "fp32", memory_budget=4_000_000) as srv:
+        small = srv.submit(tiny)
         m, _ = srv.wait(small)
-        assert m.nq == len(inps[0].q)
+        assert m.nq == 100
         bnames, bpos, bw = globule(20000, radius_for(20000), 77)
         big = foxs_engine.prepare_workload(Workload("big", bnames, bpos, linear_q(0, 0.5, 800), bw))
         rid = srv.submit(big)
