@@ -82,8 +82,10 @@ def log(msg):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    # the GPU's clocks take a few hundred milliseconds of load to settle:
+    # 300 untimed passes (~0.1 s) before the 200 timed ones (~75 ms)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--n", type=int, default=10000)
     ap.add_argument("--nq", type=int, default=500)
     ap.add_argument("--q-per-lane", type=int, default=0)

@@ -6,7 +6,7 @@
 // Jacobi-Anger expansion (a = q_c r, beta = H r),
 //   sin(a + beta x) / r = (1/r) [sin a (J_0(beta) + 2 sum_m (-1)^m J_2m(beta) T_2m(x))
 //                               + cos a (2 sum_m (-1)^m J_2m+1(beta) T_2m+1(x))],
-// a Chebyshev series whose terms beyond order P are below 1e-9 of the pair's
+// a Chebyshev series whose terms beyond order P are below 1e-8 of the pair's
 // amplitude (the host picks P from beta_max = H r_max with the bound
 // |J_p(beta)| <= (beta/2)^p / p!).
 //
@@ -32,8 +32,8 @@
 // u_0 + 2 sum_m u_2m z^m (Horner along the recurrence).  The unwanted
 // solution that the arbitrary start admits shrinks against the wanted one by
 // about z / p^2 per order on the way down; at the largest beta an order
-// admits (J_P below 5e-10 of the amplitude) z / P^2 is at most 0.04, so two
-// orders leave it under 2e-3 of J_P at order P, below 1e-12 of the pair's
+// admits (J_P below 5e-9 of the amplitude) z / P^2 is at most 0.06, so two
+// orders leave it under 4e-3 of J_P at order P, below 2e-11 of the pair's
 // amplitude, and smaller still at every lower order.  The series
 // coefficients are then v_p = u_p * 2 (-z)^floor(p/2) times a per-strip
 // factor (amp sin a / S for even p, amp b cos a / S for odd p; p = 0 takes
@@ -466,7 +466,8 @@ hipError_t launch_cheb_reduce(const ChebReduceArgs &a, hipStream_t st) {
 }
 
 // Orders the kernel is built for; the host rounds a structure's need up.
-#define PRIPPS_CHEB_ORDERS(X) X(12) X(16) X(20) X(24) X(28) X(32) X(40)
+#define PRIPPS_CHEB_ORDERS(X) \
+    X(12) X(16) X(20) X(22) X(24) X(26) X(28) X(30) X(32) X(36) X(40)
 
 hipError_t launch_cheb(int P, const TiledArgs &a, int blocks, hipStream_t st) {
     switch (P) {
@@ -501,7 +502,7 @@ static bool cheb_order_ok(int P, double beta_max) {
         tail += t;
         t *= b / (k + 1);
     }
-    return 2.0 * tail < 1e-9;
+    return 2.0 * tail < 1e-8;
 }
 
 double cheb_beta_limit(int P) {
@@ -515,10 +516,9 @@ double cheb_beta_limit(int P) {
 }
 
 int cheb_order_for(double beta_max) {
-    // smallest built order P with 2 sum_{p >= P} (beta/2)^p / p! < 1e-9
+    // smallest built order P with 2 sum_{p >= P} (beta/2)^p / p! < 1e-8
     // (|J_p(beta)| <= (beta/2)^p / p! for all p >= 0)
-    const int orders[] = {12, 16, 20, 24, 28, 32, 40};
-    for (int P : orders)
+    for (int P : kChebOrders)
         if (cheb_order_ok(P, beta_max)) return P;
     return 0;  // beta too large for any built order
 }

@@ -22,6 +22,9 @@ constexpr int kChebQ = 256;
 // Si = sum w_i s (pass 0), Sw = sum w_j s, Siw = sum w_i w_j s (pass 1,
 // hydrated j)
 constexpr int kChebSets = 4;
+// expansion orders the Chebyshev kernel is built for (ascending)
+constexpr int kChebOrders[] = {12, 16, 20, 22, 24, 26, 28, 30, 32, 36, 40};
+constexpr int kChebNumOrders = sizeof(kChebOrders) / sizeof(kChebOrders[0]);
 
 // One atom of the sorted SoA-in-AoS stream: centred coordinates and the
 // hydration weight, 16 bytes (FP32) or 32 bytes (FP64) per atom so a wave
@@ -154,7 +157,7 @@ hipError_t launch_finalize(const FinalizeArgs &a, int n_strips_total, hipStream_
 hipError_t launch_cheb(int P, const TiledArgs &a, int blocks, hipStream_t st);
 hipError_t cheb_occupancy(int P, int *blocks_per_cu);
 hipError_t launch_cheb_reduce(const ChebReduceArgs &a, hipStream_t st);
-// smallest built order whose truncation is below 1e-9 of a pair's amplitude
+// smallest built order whose truncation is below 1e-8 of a pair's amplitude
 // at beta = H r up to beta_max (0: none)
 int cheb_order_for(double beta_max);
 // the largest beta_max that order P admits
