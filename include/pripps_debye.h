@@ -88,7 +88,8 @@ typedef struct {
                              256: the FP32 Chebyshev kernel (strips of up to 256
                              q-points, any grid) */
     int32_t block;        /* pairs summed in FP32 before the FP64 carry */
-    int32_t span;         /* atoms per work item along j */
+    int32_t span;         /* atoms per work item along j; Chebyshev plans: j atoms per
+                             work unit (0: one unit per wave slot of the GPU) */
     int32_t variant;      /* FP32 inner-loop variant for the autotuner (0 = default) */
 } pripps_tuning;
 
@@ -107,8 +108,8 @@ typedef struct {
     double kernel_ms;            /* device time of the pair kernels */
     double total_ms;             /* host wall time of the build call */
     double pair_q_terms;         /* N(N-1)/2 * nq */
-    int32_t q_per_lane, block, span;
-    int64_t work_items;
+    int32_t q_per_lane, block, span;  /* span 0: Chebyshev units sized automatically */
+    int64_t work_items;          /* queue entries: items, or units of a Chebyshev plan */
     int32_t frames;              /* frames averaged (pripps_debye_frames_average), else 0 */
     int32_t frames_per_launch;   /* frames per persistent launch */
     int32_t graph_launches;      /* chunks submitted by replaying the captured HIP graph */
@@ -223,11 +224,11 @@ int pripps_debye_model_free(pripps_debye_model *model);
  * process ends; the call makes no hipMalloc, hipFree or other allocating
  * device call.  `profiles_out` (host, [6][nq], DebyeTerms order) receives
  * the profiles; no model is created.  Without an explicit tuning span, a
- * default plan whose item partials do not fit is built with longer j-spans
- * (fewer items) instead, which changes the FP32 summation order but not the
+ * plan whose work lists do not fit is built with longer j-spans (fewer items
+ * or units) instead, which changes the FP32 summation order but not the
  * precision bound; a structure that does not fit even so is refused with
- * PRIPPS_ERR_OUT_OF_MEMORY (FP32 on a uniform grid of 500 q: about 65,000
- * atoms; FP64 or longer grids: fewer).  Calls
+ * PRIPPS_ERR_OUT_OF_MEMORY (FP32 Chebyshev plan on a uniform grid of 500 q:
+ * 200,000 globule atoms fit, 300,000 do not; FP64 plans: about 65,000).  Calls
  * on one GPU share the array and are serialised by the library.  Precision:
  * FP32_COMPENSATED or FP64.
  */

@@ -758,6 +758,7 @@ struct LaunchGroup {
     int64_t n_slots = 0;  // partial slots (recurrence kernels): one per item
     // Chebyshev plans: the queue of units, and the reduction of their records
     std::vector<Unit> units;
+    int64_t unit_len = 0;  // longest unit of a member, in j atoms of a two-strip group
     int RS = 0;
     int64_t n_records = 0;
     std::vector<int32_t> chunk_recs, chunk_first;
@@ -810,7 +811,10 @@ static StructPlan resolve_plan(pripps_engine *e, const pripps_tuning *tune, int 
     if (expl && tune->span > 0) sp.span = std::max(64, tune->span);
     else if (have) sp.span = shifted_span(base, te.span_shift);
     else sp.span = base;
-    sp.span_set = (expl && tune->span > 0) || have;
+    // Chebyshev plans size their units themselves unless a span is given
+    // (0 in the build statistics)
+    sp.span_set = (expl && tune->span > 0) || (have && (Q != kChebQ || te.span_shift != 0));
+    if (Q == kChebQ && !sp.span_set) sp.span = 0;
     const int block = expl && tune->block > 0 ? tune->block
                       : have && te.block > 0  ? te.block
                       : Q == kChebQ           ? kChebBlock
@@ -872,6 +876,7 @@ static void plan_cheb_group(LaunchGroup &g, const std::vector<Prepared> &P,
         const int64_t us =
             q.span_set ? std::max<int64_t>(1, (int64_t)std::ceil(wref / q.span))
                        : std::max<int64_t>(1, std::min<int64_t>(n_waves, (int64_t)(wref / (double)kChebUnitMin)));
+        g.unit_len = std::max(g.unit_len, (int64_t)std::ceil(wref / (double)us));
         for (int u = 0; u < ng; ++u) {
             const int nk = q.unit_strips(u);
             const int strip = q.unit_first(u);
@@ -1506,13 +1511,20 @@ extern "C" int pripps_debye_build_noalloc(pripps_engine *e, const pripps_debye_i
     BuildPlan B;
     plan_build(e, P, precision, tune, user, B);
     // Without an explicit span, a plan whose partials do not fit takes longer
-    // j-spans (fewer items, so fewer partial slots) until it fits.
+    // j-spans (fewer items, so fewer partial slots) until it fits; a
+    // Chebyshev plan longer units (fewer coefficient records), up to one
+    // unit per strip group.
     const bool span_given = tune && tune->span > 0;
-    while (B.total_bytes > PRIPPS_NOALLOC_SCRATCH_BYTES && !span_given && B.sp[0].span < 16384) {
+    auto cur_span = [&](const BuildPlan &b) -> int64_t {
+        if (b.sp[0].Q != kChebQ) return b.sp[0].span;
+        return b.groups.empty() ? INT32_MAX : std::max<int64_t>(b.sp[0].span, b.groups[0].unit_len);
+    };
+    const int64_t max_span = B.sp[0].Q == kChebQ ? INT32_MAX / 2 : 16384;
+    while (B.total_bytes > PRIPPS_NOALLOC_SCRATCH_BYTES && !span_given && cur_span(B) < max_span) {
         pripps_tuning t{};
         t.q_per_lane = B.sp[0].Q;
         t.block = B.sp[0].block;
-        t.span = std::min(16384, 2 * B.sp[0].span);
+        t.span = (int32_t)std::min<int64_t>(max_span, 2 * std::max<int64_t>(64, cur_span(B)));
         t.variant = B.sp[0].variant;
         BuildPlan wider;
         plan_build(e, P, precision, &t, user, wider);
@@ -1805,11 +1817,13 @@ extern "C" int pripps_engine_autotune(pripps_engine *e, const pripps_debye_input
         }
     };
     std::vector<std::pair<Cand, double>> seen;
+    // block 0: the plan's default for Q; a Chebyshev plan at shift 0 sizes
+    // its units itself
     auto absolute = [&](const Cand &c) {
         pripps_tuning t{};
         t.q_per_lane = c.Q;
-        t.block = c.block;
-        t.span = shifted_span(default_span(p, c.Q, e->n_cu), c.shift);
+        t.block = c.block > 0 ? c.block : c.Q == kChebQ ? kChebBlock : kDefaultBlock;
+        t.span = c.Q == kChebQ && c.shift == 0 ? 0 : shifted_span(default_span(p, c.Q, e->n_cu), c.shift);
         t.variant = c.variant;
         return t;
     };
@@ -1833,7 +1847,7 @@ extern "C" int pripps_engine_autotune(pripps_engine *e, const pripps_debye_input
         return PRIPPS_OK;
     };
 
-    Cand cur{default_q(precision, p.uniform, p.nq, p.n), 0, kDefaultBlock, 3};
+    Cand cur{default_q(precision, p.uniform, p.nq, p.n), 0, 0, 3};
     double cur_ms = 0;
     rc = eval(cur, cur_ms);
     if (rc) return rc;
@@ -1867,7 +1881,7 @@ extern "C" int pripps_engine_autotune(pripps_engine *e, const pripps_debye_input
     {
         std::vector<Cand> c;
         // capped at 128: longer blocks trade accuracy on lattice-placed points
-        for (int b : {16, 32, 128}) c.push_back({cur.Q, cur.shift, b, cur.variant});
+        for (int b : {16, 32, 64, 128}) c.push_back({cur.Q, cur.shift, b, cur.variant});
         if ((rc = try_all(c))) return rc;
     }
     // 4. inner-loop variant (FP32, uniform grids)

@@ -152,7 +152,9 @@ def test_autotune_table_applies_and_keeps_batches_deterministic(foxs):
     inps = [foxs.prepare_input(p, np.arange(len(p)) % 4, np.linspace(0, 0.5, 200))
             for p in names_pos]
     best, ms = eng.autotune(inps[0], "fp32", repeats=1)
-    assert ms > 0 and best.q_per_lane in (32, 64, 128, 256) and best.span >= 64
+    # span 0: a Chebyshev plan (q_per_lane 256) that sizes its work units itself
+    assert ms > 0 and best.q_per_lane in (32, 64, 128, 256)
+    assert best.span >= 64 or (best.q_per_lane == 256 and best.span == 0)
     table = eng.tuning_table()
     assert len(table) == 1
     t0 = table[0]

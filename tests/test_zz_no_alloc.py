@@ -48,18 +48,22 @@ def test_noalloc_matches_model_build(engine, precision):
 
 
 def test_noalloc_wider_span_and_refusal(engine, foxs):
-    """The 10,000-atom solvent globule at nq = 500: its default plan's partials
-    exceed the fixed scratch, so the call takes longer j-spans and still
-    meets the FP32 bound against the ordinary build; with an explicit short
-    span it cannot fit and is refused."""
+    """The 10,000-atom solvent globule at nq = 500.  Its default (Chebyshev)
+    plan fits the fixed scratch and meets the FP32 bound against the ordinary
+    build.  The 128-q recurrence plan's item partials (about 77 MB) do not:
+    without a span the call takes longer j-spans until they fit and still
+    meets the bound; with an explicit short span it is refused."""
     from pripps_amd._native import tuning
     from pripps_amd.errors import OutOfDeviceMemoryError, from_status
     from pripps_amd.synth import solvent10k
     wl = solvent10k()
     inp = foxs.prepare_workload(wl)
+    ref = engine.build(inp, "fp32").profiles()
     rc, got = _noalloc(engine, inp, "fp32")
     assert rc == 0
-    ref = engine.build(inp, "fp32").profiles()
+    assert envelope_error(got, ref).max() <= 1e-5
+    rc, got = _noalloc(engine, inp, "fp32", tuning(q_per_lane=128))
+    assert rc == 0
     assert envelope_error(got, ref).max() <= 1e-5
     rc, _ = _noalloc(engine, inp, "fp32", tuning(q_per_lane=128, span=64))
     with pytest.raises(OutOfDeviceMemoryError, match="fixed scratch"):
