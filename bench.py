@@ -49,10 +49,10 @@ FLOP_PER_TERM_SOLVENT = 14.0
 #   valu_wave_instr: SQ_INSTS_VALU per dispatch; with the dispatch's
 #          GRBM_GUI_ACTIVE cycles per XCD it gives the issue rate.
 COUNTERS = {
-    ("debye_cheb_kernel<28>", 256, 192, 8568): {
-        "bytes": (2134.0 * 2.0 + 103200.0 * 1.0) * 1024,
-        "executed_flop_per_term": 64.0 * (2 * 1.630e8 + 1.798e7 + 1.216e8 + 3.459e6) / 2.49975e10,
-        "valu_wave_instr": 3.534e8, "cycles_per_xcd": 9.677e6 / 8,
+    ("debye_cheb_kernel<26>", 256, 48, 0, 3072): {
+        "bytes": (1354.5 * 2.0 + 7774.0 * 1.0) * 1024,
+        "executed_flop_per_term": 64.0 * (2 * 1.0896e8 + 7.8716e6 + 7.3205e7 + 1.7296e6) / 2.49975e10,
+        "valu_wave_instr": 2.1420e8, "cycles_per_xcd": 6.5117e6 / 8,
         "profile": "profiles/debye_cheb_f32_solvent10k.md"},
 }
 F64_EXECUTED_FLOP_PER_TERM = 6.71
@@ -255,7 +255,7 @@ def main():
     kernel = (f"debye_cheb_kernel<{st.cheb_order}>" if st.q_per_lane == 256 else
               "debye_dual_kernel" if st.q_per_lane == 128 else
               "debye_tiled_kernel<float,%d,true,3,true>" % st.q_per_lane)
-    ctr = COUNTERS.get((kernel, st.q_per_lane, st.span, st.work_items))
+    ctr = COUNTERS.get((kernel, st.q_per_lane, st.block, st.span, st.work_items))
     roofline = {"bound": "valu", "unit": "TFLOP/s", "peak": FP32_PEAK_TFLOPS, "kernel": kernel,
                 "kernel_ms": pair_ms, "traffic": None, "achieved": None, "frac": None,
                 "issue_frac": None,

@@ -10,14 +10,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <array>
 #include <climits>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <atomic>
-#include <map>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -837,6 +835,23 @@ static StructPlan resolve_plan(pripps_engine *e, const pripps_tuning *tune, int 
 // or in any batch.  The queue runs the units costliest first (a stable sort).
 constexpr int64_t kChebUnitMin = 48;
 
+// The same cut for two structures: same atoms per class, tiles, strips and
+// order (frames of one topology, say).
+static bool same_cut(const Prepared &a, const StructPlan &qa, const Prepared &b, const StructPlan &qb) {
+    if (a.n != b.n || a.nq != b.nq || a.n_classes != b.n_classes || qa.P != qb.P ||
+        qa.span_set != qb.span_set || qa.span != qb.span || qa.group_first != qb.group_first ||
+        a.tiles.size() != b.tiles.size())
+        return false;
+    for (int c = 0; c <= a.n_classes; ++c)
+        if (a.cls_start[c] != b.cls_start[c]) return false;
+    for (int c = 0; c < a.n_classes; ++c)
+        if (a.cls_kind[c] != b.cls_kind[c] || a.cls_wet[c] != b.cls_wet[c]) return false;
+    for (size_t t = 0; t < a.tiles.size(); ++t)
+        if (a.tiles[t].x != b.tiles[t].x || a.tiles[t].y != b.tiles[t].y || a.tiles[t].z != b.tiles[t].z)
+            return false;
+    return true;
+}
+
 static void plan_cheb_group(LaunchGroup &g, const std::vector<Prepared> &P,
                             const std::vector<StructPlan> &sp, int n_waves) {
     struct Pending {
@@ -846,22 +861,57 @@ static void plan_cheb_group(LaunchGroup &g, const std::vector<Prepared> &P,
     };
     std::vector<Pending> pend;
     g.items.clear();
+    int prev_s = -1;
+    size_t prev_items0 = 0, prev_items1 = 0, prev_pend0 = 0, prev_pend1 = 0;
     for (int s : g.members) {
         const Prepared &p = P[s];
         const StructPlan &q = sp[s];
         const int ng = q.n_units(p.nq);
         if (ng == 0 || p.tiles.empty()) continue;
-        // hydrated atoms before j, from the class ranges
-        auto wet_before = [&](int32_t j) {
-            int64_t w = 0;
-            for (int c = 0; c < p.n_classes; ++c)
-                if (p.cls_wet[c]) {
-                    const int32_t lo = p.cls_start[c], hi = std::min(p.cls_start[c + 1], j);
-                    if (hi > lo) w += hi - lo;
+        if (prev_s >= 0 && same_cut(P[prev_s], sp[prev_s], p, q)) {
+            // copy the previous structure's items and units
+            const int32_t shift = (int32_t)(g.items.size() - prev_items0);
+            for (size_t i = prev_items0; i < prev_items1; ++i) {
+                Item it = g.items[i];
+                it.s = s;
+                g.items.push_back(it);
+            }
+            for (size_t x = prev_pend0; x < prev_pend1; ++x) {
+                Pending pd = pend[x];
+                pd.s = s;
+                pd.u.item0 += shift;
+                pd.u.item1 += shift;
+                pend.push_back(pd);
+            }
+            prev_items0 = prev_items1;
+            prev_items1 = g.items.size();
+            prev_pend0 = prev_pend1;
+            prev_pend1 = pend.size();
+            prev_s = s;
+            continue;
+        }
+        const size_t items0 = g.items.size(), pend0 = pend.size();
+        // the class segments from j on, each j costing 1 (2 if hydrated)
+        auto cost = [&](int32_t a, int32_t b) {
+            int64_t c = b - a;
+            for (int k = 0; k < p.n_classes; ++k)
+                if (p.cls_wet[k]) {
+                    const int32_t lo = std::max(a, p.cls_start[k]), hi = std::min(b, p.cls_start[k + 1]);
+                    if (hi > lo) c += hi - lo;
                 }
-            return w;
+            return c;
         };
-        auto cost = [&](int32_t a, int32_t b) { return (int64_t)(b - a) + wet_before(b) - wet_before(a); };
+        // the first j1 > a with cost(a, j1) >= need (need > 0, below cost(a, n))
+        auto advance = [&](int32_t a, int64_t need) {
+            for (int k = 0; k < p.n_classes; ++k) {
+                const int32_t lo = std::max(a, p.cls_start[k]), hi = p.cls_start[k + 1];
+                if (hi <= lo) continue;
+                const int64_t w = p.cls_wet[k] ? 2 : 1;
+                if ((hi - lo) * w >= need) return (int32_t)(lo + (need + w - 1) / w);
+                need -= (hi - lo) * w;
+            }
+            return p.n;
+        };
         int64_t W = 0;
         for (const int4 &t : p.tiles) W += cost(t.x, p.n);
         if (W <= 0) continue;
@@ -884,28 +934,27 @@ static void plan_cheb_group(LaunchGroup &g, const std::vector<Prepared> &P,
             ug = std::min(ug, W);
             size_t t = 0;
             int32_t j = p.tiles[0].x;
+            int64_t rest = cost(j, p.n);  // cost of the current tile from j on
             int64_t done = 0, prev = 0;
             for (int64_t k = 0; k < ug && t < p.tiles.size(); ++k) {
                 const int64_t target = k + 1 == ug ? W : W * (k + 1) / ug;
                 const int32_t item0 = (int32_t)g.items.size();
                 while (done < target && t < p.tiles.size()) {
                     const int4 &tl = p.tiles[t];
-                    const int64_t rest = cost(j, p.n);
                     int32_t j1 = p.n;
+                    int64_t c = rest;
                     if (done + rest > target) {
-                        // the first j1 with cost(j, j1) >= target - done
-                        int32_t lo = j + 1, hi = p.n;
-                        while (lo < hi) {
-                            const int32_t mid = lo + (hi - lo) / 2;
-                            if (cost(j, mid) >= target - done) hi = mid;
-                            else lo = mid + 1;
-                        }
-                        j1 = lo;
+                        j1 = advance(j, target - done);
+                        c = cost(j, j1);
                     }
                     g.items.push_back(Item{s, tl.x, tl.y, tl.z, j, j1, strip, nk - 1});
-                    done += cost(j, j1);
+                    done += c;
+                    rest -= c;
                     if (j1 == p.n) {
-                        if (++t < p.tiles.size()) j = p.tiles[t].x;
+                        if (++t < p.tiles.size()) {
+                            j = p.tiles[t].x;
+                            rest = cost(j, p.n);
+                        }
                     } else {
                         j = j1;
                     }
@@ -916,65 +965,98 @@ static void plan_cheb_group(LaunchGroup &g, const std::vector<Prepared> &P,
                 prev = done;
             }
         }
+        prev_s = s;
+        prev_items0 = items0;
+        prev_items1 = g.items.size();
+        prev_pend0 = pend0;
+        prev_pend1 = pend.size();
     }
-    std::vector<int32_t> order(pend.size());
-    for (size_t x = 0; x < order.size(); ++x) order[x] = (int32_t)x;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int32_t a, int32_t b) { return pend[a].cost > pend[b].cost; });
+    // costliest first, ties in generation order
+    std::vector<std::pair<double, int32_t>> order(pend.size());
+    for (size_t x = 0; x < order.size(); ++x) order[x] = {-pend[x].cost, (int32_t)x};
+    std::sort(order.begin(), order.end());
     g.units.resize(pend.size());
-    for (size_t x = 0; x < order.size(); ++x) g.units[x] = pend[order[x]].u;
+    for (size_t x = 0; x < order.size(); ++x) g.units[x] = pend[order[x].second].u;
     // Records: one per class run of each item, unit by unit in queue order
     // (the order in which the kernel numbers them from the unit's rec0).
-    // Combos (structure, group's first strip, i kind, j kind) list their
-    // records in that order; a structure's lists do not depend on the batch.
+    // Combos (structure, strip group, i kind, j kind), numbered in that
+    // order, list their records in queue order (a counting sort), so a
+    // structure's lists do not depend on the batch.
     g.RS = (2 * g.P + 63) / 64 * 64;
-    std::map<std::array<int32_t, 4>, std::vector<int32_t>> combos;
-    int64_t rec = 0;
+    const int nm = (int)g.members.size();
+    std::vector<int> local(P.size(), -1);
+    std::vector<int64_t> cbase(nm + 1, 0);
+    for (int k = 0; k < nm; ++k) {
+        const int s = g.members[k];
+        local[s] = k;
+        cbase[k + 1] = cbase[k] + (int64_t)sp[s].n_units(P[s].nq) * P[s].K * P[s].K;
+    }
+    // strip -> group index of each member (strip groups are contiguous)
+    std::vector<std::vector<int32_t>> group_of(nm);
+    for (int k = 0; k < nm; ++k) {
+        const StructPlan &q = sp[g.members[k]];
+        const int ns = q.n_strips(P[g.members[k]].nq);
+        group_of[k].assign(std::max(ns, 1), 0);
+        for (int u = 0; u < q.n_units(P[g.members[k]].nq); ++u)
+            for (int st = q.unit_first(u); st < q.unit_first(u) + q.unit_strips(u); ++st) group_of[k][st] = u;
+    }
+    std::vector<int32_t> rec_combo;
     for (Unit &u : g.units) {
-        u.rec0 = (int32_t)rec;
+        u.rec0 = (int32_t)rec_combo.size();
         for (int32_t i = u.item0; i < u.item1; ++i) {
             const Item &it = g.items[i];
             const Prepared &p = P[it.s];
+            const int k = local[it.s];
+            const int64_t base = cbase[k] + ((int64_t)group_of[k][it.strip] * p.K + it.ikind) * p.K;
             for (int c = 0; c < p.n_classes; ++c) {
                 const int32_t lo = std::max(it.j0, p.cls_start[c]);
                 const int32_t hi = std::min(it.j1, p.cls_start[c + 1]);
-                if (lo >= hi) continue;
-                combos[{it.s, it.strip, it.ikind, (int32_t)p.cls_kind[c]}].push_back((int32_t)rec++);
+                if (lo < hi) rec_combo.push_back((int32_t)(base + p.cls_kind[c]));
             }
         }
     }
-    g.n_records = rec;
-    // stage 1 chunks, combo by combo in key order; stage 2 blocks per strip
-    // and combo; stage 3 blocks per strip (every strip, with or without work)
-    g.chunk_recs.clear();
-    g.chunk_first.assign(1, 0);
-    std::map<std::array<int32_t, 4>, std::array<int32_t, 2>> combo_chunks;
-    for (const auto &kv : combos) {
-        const int32_t c0 = (int32_t)g.chunk_first.size() - 1;
-        const auto &lst = kv.second;
-        for (size_t x = 0; x < lst.size(); x += kFinChunk) {
-            const size_t e = std::min(lst.size(), x + (size_t)kFinChunk);
-            g.chunk_recs.insert(g.chunk_recs.end(), lst.begin() + x, lst.begin() + e);
-            g.chunk_first.push_back((int32_t)g.chunk_recs.size());
-        }
-        combo_chunks[kv.first] = {c0, (int32_t)g.chunk_first.size() - 1};
+    g.n_records = (int64_t)rec_combo.size();
+    const int64_t n_combos = cbase[nm];
+    std::vector<int32_t> cptr(n_combos + 1, 0);
+    for (int32_t c : rec_combo) cptr[c + 1]++;
+    for (int64_t c = 0; c < n_combos; ++c) cptr[c + 1] += cptr[c];
+    std::vector<int32_t> crec(rec_combo.size());
+    {
+        std::vector<int32_t> fill(cptr.begin(), cptr.end() - 1);
+        for (size_t r = 0; r < rec_combo.size(); ++r) crec[fill[rec_combo[r]]++] = (int32_t)r;
     }
+    // stage 1 chunks, combo by combo; stage 2 blocks per strip and non-empty
+    // combo of its group; stage 3 blocks per strip (every strip, with or
+    // without work)
+    g.chunk_recs.assign(crec.begin(), crec.end());
+    g.chunk_first.assign(1, 0);
+    std::vector<int32_t> c_first(n_combos + 1, 0);
+    for (int64_t c = 0; c < n_combos; ++c) {
+        c_first[c] = (int32_t)g.chunk_first.size() - 1;
+        for (int32_t x = cptr[c]; x < cptr[c + 1]; x += kFinChunk)
+            g.chunk_first.push_back(std::min(cptr[c + 1], x + kFinChunk));
+    }
+    c_first[n_combos] = (int32_t)g.chunk_first.size() - 1;
     g.sb_a.clear();
     g.sb_b.clear();
     g.st.clear();
-    for (int s : g.members) {
+    for (int k = 0; k < nm; ++k) {
+        const int s = g.members[k];
         const StructPlan &q = sp[s];
+        const int K = P[s].K;
         const int ng = q.n_units(P[s].nq);
         for (int u = 0; u < ng; ++u) {
             const int gf = q.unit_first(u);
-            for (int k = 0; k < q.unit_strips(u); ++k) {
+            for (int ks = 0; ks < q.unit_strips(u); ++ks) {
                 const int32_t b0 = (int32_t)g.sb_a.size();
-                for (auto it = combo_chunks.lower_bound({s, gf, INT32_MIN, INT32_MIN});
-                     it != combo_chunks.end() && it->first[0] == s && it->first[1] == gf; ++it) {
-                    g.sb_a.push_back(int4{s, gf + k, k, it->first[2]});
-                    g.sb_b.push_back(int4{it->first[3], it->second[0], it->second[1], 0});
-                }
-                g.st.push_back(int4{s, gf + k, b0, (int32_t)g.sb_a.size()});
+                for (int ik = 0; ik < K; ++ik)
+                    for (int jk = 0; jk < K; ++jk) {
+                        const int64_t c = cbase[k] + ((int64_t)u * K + ik) * K + jk;
+                        if (cptr[c + 1] == cptr[c]) continue;
+                        g.sb_a.push_back(int4{s, gf + ks, ks, ik});
+                        g.sb_b.push_back(int4{jk, c_first[c], c_first[c + 1], 0});
+                    }
+                g.st.push_back(int4{s, gf + ks, b0, (int32_t)g.sb_a.size()});
             }
         }
     }
@@ -1053,7 +1135,16 @@ static void plan_build(pripps_engine *e, const std::vector<Prepared> &P, int pre
     B.sp.assign(ns, StructPlan{});
     if (!B.plain) {
         for (int s = 0; s < ns; ++s) {
-            B.sp[s] = resolve_plan(e, tune, precision, P[s]);
+            // consecutive structures with the same plan inputs (frames of one
+            // topology) take the plan of the one before
+            const Prepared &a = P[s];
+            const Prepared *b = s > 0 ? &P[s - 1] : nullptr;
+            if (b && a.n == b->n && a.nq == b->nq && a.uniform == b->uniform && a.rmax == b->rmax &&
+                a.tiles.size() == b->tiles.size() &&
+                std::memcmp(a.q.data(), b->q.data(), (size_t)a.nq * sizeof(double)) == 0)
+                B.sp[s] = B.sp[s - 1];
+            else
+                B.sp[s] = resolve_plan(e, tune, precision, P[s]);
             LaunchGroup *g = nullptr;
             for (auto &x : B.groups)
                 if (x.Q == B.sp[s].Q && x.uniform == P[s].uniform && x.variant == B.sp[s].variant &&
