@@ -447,9 +447,9 @@ def _globule_job(job):
 
 def _batch_line(eng, ff, args):
     """The batched configuration inside the default run: the structures in
-    one persistent FP32 launch (resident inputs, `recompute`), against an
-    evenly spaced sample of them built one at a time on the plain FP64 path
-    (the rate extrapolated by pair-q terms)."""
+    one persistent FP32 launch per kernel shape (resident inputs,
+    `recompute`), against an evenly spaced sample of them built one at a time
+    on the plain FP64 path (the rate extrapolated by pair-q terms)."""
     import numpy as np
     t = time.perf_counter()
     wls = _make_batch(args.batch_count)
@@ -470,7 +470,7 @@ def _batch_line(eng, ff, args):
         del m
     plain_rate = plain_terms / (plain_ms * 1e-3)
     rec = {"workload": f"batch{len(wls)}: globules N log-uniform [500,20000], jitter 0.5 A, own q "
-                       "grid nq U[100,1000] qmax U[0.3,0.8], solvent; one persistent FP32 launch",
+                       "grid nq U[100,1000] qmax U[0.3,0.8], solvent; one persistent FP32 launch per kernel shape",
            "structures": len(wls), "pair_q_terms": terms, "ms_per_step": ms, "pair_kernel_ms": pair_ms,
            "terms_per_s": terms / (ms * 1e-3), "first_build_wall_ms": first_wall,
            "fp64_plain_sequential": {"sample": int(len(sel)), "sample_terms": plain_terms,
@@ -482,8 +482,8 @@ def _batch_line(eng, ff, args):
 
 
 def bench_batch512(args):
-    """512 structures of uneven size and q grid in one persistent launch,
-    against the plain FP64 path launched once per structure."""
+    """512 structures of uneven size and q grid in one persistent launch per
+    kernel shape, against the plain FP64 path launched once per structure."""
     import numpy as np
     eng, ff = _engine_and_ff()
     t = time.perf_counter()

@@ -138,8 +138,9 @@ int pripps_debye_model_build(pripps_engine *engine, const pripps_debye_input *in
                              pripps_debye_model **out);
 
 /*
- * Build many models in one persistent launch (structures of uneven size and
- * q grid share one device work queue).  `out` receives n handles.  The
+ * Build many models in one persistent launch per kernel shape (structures of
+ * uneven size and q grid share one device work queue; FP32 Chebyshev plans
+ * run one launch per expansion order).  `out` receives n handles.  The
  * reference has no batched entry point; its multi-frame driver is the serial
  * loop of trajectory_average (src/trajectory.rs:167-251).
  */
@@ -228,7 +229,8 @@ int pripps_debye_model_free(pripps_debye_model *model);
  * or units) instead, which changes the FP32 summation order but not the
  * precision bound; a structure that does not fit even so is refused with
  * PRIPPS_ERR_OUT_OF_MEMORY (FP32 Chebyshev plan on a uniform grid of 500 q:
- * 200,000 globule atoms fit, 300,000 do not; FP64 plans: about 65,000).  Calls
+ * 200,000 globule atoms fit, 300,000 do not; FP64: 55,000 fit, 65,000 do
+ * not).  Calls
  * on one GPU share the array and are serialised by the library.  Precision:
  * FP32_COMPENSATED or FP64.
  */

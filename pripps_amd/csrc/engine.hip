@@ -692,11 +692,9 @@ int default_q(int precision, bool uniform, int nq, int n_atoms) {
 // into the same items alone or in any batch, and its profiles are bitwise
 // the same either way.
 //
-// Chebyshev plans (Q = kChebQ) run three waves per SIMD and cost about 5x
-// less per j atom, so an item is short; their span aims at about three
-// items per wave slot of the GPU (64..8192): on the 10,000-atom, 500-q
-// globule span 192 (8,568 items) measured 0.520 ms against 0.593 ms at 256
-// (6,464) and 0.525 at 128 (12,768, more partial slots to reduce).
+// Chebyshev plans (Q = kChebQ) cut their work into units themselves (span
+// 0, plan_cheb_group); the span below, about three units per wave slot of
+// the GPU (64..8192), is only the base that a tuning-table shift scales.
 int default_span(const Prepared &p, int Q, int n_cu, int n_units = 0) {
     const double strips = n_units > 0 ? (double)n_units : (double)((p.nq + Q - 1) / Q);
     const double work = (double)p.tiles.size() * 0.5 * p.n * strips;
