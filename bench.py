@@ -48,11 +48,14 @@ FLOP_PER_TERM_SOLVENT = 14.0
 #          pair-q terms;
 #   valu_wave_instr: SQ_INSTS_VALU per dispatch; with the dispatch's
 #          GRBM_GUI_ACTIVE cycles per XCD it gives the issue rate.
+# Keyed by the library's version string (pripps_version(), changed whenever
+# the kernel's instruction stream changes) and the plan, so that a library or
+# plan the counters were not measured on finds no entry.
 COUNTERS = {
-    ("debye_cheb_kernel<26>", 256, 48, 0, 3072): {
-        "bytes": (1354.5 * 2.0 + 7774.0 * 1.0) * 1024,
-        "executed_flop_per_term": 64.0 * (2 * 1.0896e8 + 7.8716e6 + 7.3205e7 + 1.7296e6) / 2.49975e10,
-        "valu_wave_instr": 2.1420e8, "cycles_per_xcd": 6.5117e6 / 8,
+    ("pripps_amd 0.2.0 (gfx950)", "debye_cheb_kernel<26>", 256, 48, 0, 3072): {
+        "bytes": (1272.6 * 2.0 + 7774.1 * 1.0) * 1024,
+        "executed_flop_per_term": 64.0 * (2 * 9.9451e7 + 1.9979e7 + 3.1695e7 + 2.5944e6) / 2.49975e10,
+        "valu_wave_instr": 1.7960e8, "cycles_per_xcd": 5.1347e6 / 8,
         "profile": "profiles/debye_cheb_f32_solvent10k.md"},
 }
 F64_EXECUTED_FLOP_PER_TERM = 6.71
@@ -255,7 +258,8 @@ def main():
     kernel = (f"debye_cheb_kernel<{st.cheb_order}>" if st.q_per_lane == 256 else
               "debye_dual_kernel" if st.q_per_lane == 128 else
               "debye_tiled_kernel<float,%d,true,3,true>" % st.q_per_lane)
-    ctr = COUNTERS.get((kernel, st.q_per_lane, st.block, st.span, st.work_items))
+    lib_version = _native.load().pripps_version().decode()
+    ctr = COUNTERS.get((lib_version, kernel, st.q_per_lane, st.block, st.span, st.work_items))
     roofline = {"bound": "valu", "unit": "TFLOP/s", "peak": FP32_PEAK_TFLOPS, "kernel": kernel,
                 "kernel_ms": pair_ms, "traffic": None, "achieved": None, "frac": None,
                 "issue_frac": None,

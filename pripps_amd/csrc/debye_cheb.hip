@@ -12,32 +12,49 @@
 //
 // Work item: a 64-atom i-tile (one lane per i atom) against a j-span, for a
 // group of one or two strips that share H; a wave takes a unit (a run of
-// items of one group) from the queue and sums it into the unit's slots.  Sharing H shares beta, so the
-// pair's Bessel weights -- most of the per-pair work -- serve both strips;
-// only sin a, cos a and the P accumulations are per strip.  Each lane adds
-// the coefficients of its pairs in FP32 over a chunk of up to `block` j
-// atoms of one class; then the wave reduce-scatters them (lane f receives
-// coefficient f summed over the 64 lanes) and adds them to FP64 sums in LDS.
-// When the j class changes, and at the end of the item, every lane evaluates
-// the FP64 series at its own q-points (Clenshaw), multiplies in the form
-// factors of the i kind and the j class, and adds the result to the item's
-// partial slot.  By linearity that equals summing the evaluated chunks.
+// items of one group) and sums it into the unit's coefficient records.
+// Sharing H shares beta, so the pair's Bessel weights -- most of the per-pair
+// work -- serve both strips; only sin a, cos a and the P accumulations are
+// per strip.  Each lane adds the coefficients of its pairs in FP32 over a
+// chunk of up to `block` j atoms of one class; then the wave reduce-scatters
+// them (lane f receives coefficient f summed over the 64 lanes) and adds them
+// to FP64 sums in LDS, which go to a record when the j class changes and at
+// the end of the item.  The reduction kernels below sum the records per
+// (i kind, j kind), evaluate the series at the q-points and multiply in the
+// form factors.
 //
-// Bessel weights per pair: Miller's backward recurrence on the scaled values
-// u_p = J_p(beta) (2 / beta)^p, u_{p-1} = p u_p - z u_{p+1} with z = (beta/2)^2,
-// started at N = P + 2 from (u_{N+1}, u_N) = (0, 2^-100).  The coefficients are
-// exact integers, so no rounded constant enters the weights: their rounding
-// errors vary erratically from pair to pair instead of following r.  The scale
-// comes from Neumann's sum 1 = J_0 + 2 sum_m J_2m, which in u is
-// u_0 + 2 sum_m u_2m z^m (Horner along the recurrence).  The unwanted
-// solution that the arbitrary start admits shrinks against the wanted one by
-// about z / p^2 per order on the way down; at the largest beta an order
-// admits (J_P below 5e-9 of the amplitude) z / P^2 is at most 0.06, so two
-// orders leave it under 4e-3 of J_P at order P, below 2e-11 of the pair's
-// amplitude, and smaller still at every lower order.  The series
-// coefficients are then v_p = u_p * 2 (-z)^floor(p/2) times a per-strip
-// factor (amp sin a / S for even p, amp b cos a / S for odd p; p = 0 takes
-// half the even factor), with v_p formed once for both strips.
+// Bessel weights per pair: Miller's backward recurrence.  With z = (beta/2)^2
+// and u_p = J_p(beta) (2 / beta)^p the recurrence reads
+// u_{p-1} = p u_p - z u_{p+1}; the kernel runs it on w_p = 2 (-z)^floor(p/2) u_p,
+// which are the series weights themselves (2 (-1)^m J_2m for even p = 2m,
+// 2 (-1)^m J_2m+1 (2 / beta) for odd p = 2m + 1, times one scale per pair):
+//   w_{p-1} = (2m + 1) w_p + w_{p+1}   (p = 2m + 1 odd),
+//   w_{p-1} = (-p / z) w_p + w_{p+1}   (p even),
+// one FMA per order and one multiply per two.  Started at N = P + 2 from
+// (w_{N+1}, w_N) = (0, s), it has the scale of s; Neumann's sum
+// 1 = J_0 + 2 sum_m J_2m, in w: S = w_0 / 2 + sum_{m >= 1} (-1)^m w_2m, removes
+// it.  The unwanted solution that the arbitrary start admits shrinks against
+// the wanted one by about z / p^2 per order on the way down; at the largest
+// beta an order admits (J_P below 5e-9 of the amplitude) z / P^2 is at most
+// 0.06, so two orders leave it under 4e-3 of J_P at order P, below 2e-11 of
+// the pair's amplitude, and smaller still at every lower order.
+//
+// Range: w_0 / w_N grows like N! (2 / beta)^N as beta falls, beyond FP32 for
+// small beta.  Every |w_p| is kept within [2^-126, 2^50] -- so that the
+// weights amp sin a / S below stay normal numbers -- by three starts that
+// every lane runs through (ChebLimits): s = 2^-113 at order N for beta above a
+// threshold t1 (the largest growth from N is 2^163 there) and 0 below it; a
+// restart of 2^-126 added at order 14 (for N > 16) for beta above t2 (growth
+// from order 14 below 2^176), 0 below; and a restart of 2^-126 at order 4 for
+// every lane.  Below t1 the sequence starts at order 14, where J_14(t1) is
+// below 1e-10 of the amplitude, and below t2 at order 4 (J_4(t2) < 1e-10).
+// Where a start is on, a restart added below it changes the sequence by less
+// than 2^-13 J_N / J_L, far below the FP32 rounding.  z is clamped below at a
+// floor that keeps -p / z finite and changes no weight.
+//
+// The series coefficients are w_p times a per-strip factor: amp sin a / S for
+// even p, amp b cos a / S for odd p (b = beta / 2), with w_p formed once for
+// both strips; coefficient 0 is added in full and the reduction halves it.
 //
 // The phases use the same split constants as the recurrence kernels: q_c / 2 pi
 // and H / 2 as float hi + lo parts, r and 1/r from residual-corrected square
@@ -46,6 +63,10 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <cmath>
+#include <mutex>
+
 #include "internal.h"
 #include "pair_math.h"
 
@@ -53,8 +74,6 @@ namespace pripps {
 
 #define PRIPPS_LDS __attribute__((address_space(3)))
 #define PRIPPS_GLOBAL __attribute__((address_space(1)))
-
-constexpr int kChebM = kChebQ / 64;  // q-points per lane and strip
 
 __device__ __forceinline__ int fbits(float v) { return __builtin_bit_cast(int, v); }
 __device__ __forceinline__ float bitsf(int v) { return __builtin_bit_cast(float, v); }
@@ -122,14 +141,26 @@ __device__ __forceinline__ float reduce_scatter(F in) {
     return o[0];
 }
 
+// Starts of the recurrence (see the file comment): the start at order
+// N = P + 2 (2^-113), a restart at order kChebMid (2^-126, for N > 16) and a
+// fixed restart 2^-126 at order kChebLow that every lane receives.
+constexpr float kChebStartTop = 0x1p-113f;
+constexpr float kChebStartLow = 0x1p-126f;
+constexpr int kChebMid = 14;
+constexpr int kChebLow = 4;
+template <int P>
+constexpr bool kChebThree = P + 2 > 16;
+
 // Coefficients of one pair for K strips of one group, added to acc[k][0..P):
 // see the file comment.  amp = take / r (0 for padding lanes and pairs on or
-// below the diagonal).
+// below the diagonal).  Coefficient 0 is added in full (w_0 = 2 u_0); the
+// reduction halves it.
 template <int P, int K>
 __device__ __forceinline__ void cheb_pair(float r, float amp, const float (&qc_hi)[K],
                                           const float (&qc_lo)[K], float hh_hi, float hh_lo,
-                                          float (&acc)[K][P]) {
+                                          const ChebLimits &lim, float (&acc)[K][P]) {
     constexpr int N = P + 2;
+    constexpr int LOW = kChebLow;
     float sa[K], ca[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -137,22 +168,34 @@ __device__ __forceinline__ void cheb_pair(float r, float amp, const float (&qc_h
         sincos_rev(arev, sa[k], ca[k]);
     }
     const float b = __builtin_fmaf(hh_hi, r, hh_lo * r);  // beta / 2
-    const float z = b * b;
-    // pass 1: u_{p-1} = p u_p - z u_{p+1}, downward from u_N; -z u_{p+1} is
-    // known a step early, so the chain takes one FMA per order
-    float u[P];
-    float un = 0.0f, uc = 7.888609052e-31f;  // 2^-100
-    float h = (N % 2 == 0) ? uc : 0.0f;
+    const float z = fmaxf(b * b, lim.z_floor);
+    float rz = __builtin_amdgcn_rcpf(z);
+    rz = __builtin_fmaf(rz, __builtin_fmaf(-z, rz, 1.0f), rz);
+    const float s_top = __builtin_amdgcn_fmed3f(__builtin_fmaf(z, lim.k1, lim.b1), 0.0f, kChebStartTop);
+    float s_mid = 0.0f;
+    if constexpr (kChebThree<P>)
+        s_mid = __builtin_amdgcn_fmed3f(__builtin_fmaf(z, lim.k2, lim.b2), 0.0f, kChebStartLow);
+    // w_{p-1} = (-p / z) w_p + w_{p+1} (p even), p w_p + w_{p+1} (p odd), down
+    // from (w_{N+1}, w_N) = (0, s_top); the restarts enter the addend.  The
+    // Neumann sum runs along: E = sum_{m >= 1} (-1)^m w_2m.
+    float w[P];
+    float wn = 0.0f, wc = s_top;
+    float E = ((N / 2) % 2 == 0) ? s_top : -s_top;
 #pragma unroll
     for (int p = N; p >= 1; --p) {
-        const float up = __builtin_fmaf(float(p), uc, -z * un);
-        un = uc;
-        uc = up;
-        if (p - 1 < P) u[p - 1] = up;
-        if ((p - 1) % 2 == 0) h = __builtin_fmaf(h, z, up);
+        float add = wn;
+        if (p - 1 == LOW) add += kChebStartLow;
+        if constexpr (kChebThree<P>)
+            if (p - 1 == kChebMid) add += s_mid;
+        const float wm = (p % 2 == 0) ? __builtin_fmaf(float(-p) * rz, wc, add)
+                                      : __builtin_fmaf(float(p), wc, add);
+        if (p - 1 < P) w[p - 1] = wm;
+        if ((p - 1) % 2 == 0 && p - 1 >= 2) E = ((p - 1) / 2 % 2 == 0) ? E + wm : E - wm;
+        wn = wc;
+        wc = wm;
     }
-    // Neumann normalisation: S = u_0 + 2 sum_m u_2m z^m = 2 h - u_0
-    const float S = __builtin_fmaf(2.0f, h, -u[0]);
+    // Neumann normalisation: S = u_0 + 2 sum_m u_2m z^m = w_0 / 2 + E
+    const float S = __builtin_fmaf(0.5f, w[0], E);
     float inv = __builtin_amdgcn_rcpf(S);
     inv = __builtin_fmaf(inv, __builtin_fmaf(-S, inv, 1.0f), inv);
     const float a0 = amp * inv;
@@ -163,19 +206,11 @@ __device__ __forceinline__ void cheb_pair(float r, float amp, const float (&qc_h
         e[k] = a0 * sa[k];   // even orders: amp sin a / S
         o[k] = a0b * ca[k];  // odd orders: amp (beta/2) cos a / S
     }
-    // pass 2: c_0 = u_0 e, c_p = u_p zeta_m (e or o) with zeta_m = 2 (-z)^m,
-    // m = floor(p / 2)
 #pragma unroll
-    for (int k = 0; k < K; ++k) acc[k][0] = __builtin_fmaf(u[0], e[k], acc[k][0]);
-    float zeta = 2.0f;
-#pragma unroll
-    for (int p = 1; p < P; ++p) {
-        if (p >= 2 && p % 2 == 0) zeta = -z * zeta;
-        const float v = u[p] * zeta;
+    for (int p = 0; p < P; ++p)
 #pragma unroll
         for (int k = 0; k < K; ++k)
-            acc[k][p] = __builtin_fmaf(v, (p % 2 == 0) ? e[k] : o[k], acc[k][p]);
-    }
+            acc[k][p] = __builtin_fmaf(w[p], (p % 2 == 0) ? e[k] : o[k], acc[k][p]);
 }
 
 // One chunk's pairs: j in [jb, je), all of one class, summed per lane into
@@ -185,7 +220,8 @@ template <int P, int K, bool MASK, bool BYW>
 __device__ __forceinline__ void cheb_chunk(const Atom<float> *__restrict__ atoms, int jb, int je,
                                            const Atom<float> &ai, int iidx, float lane_take,
                                            const float (&qc_hi)[K], const float (&qc_lo)[K],
-                                           float hh_hi, float hh_lo, float (&acc)[K][P]) {
+                                           float hh_hi, float hh_lo, const ChebLimits &lim,
+                                           float (&acc)[K][P]) {
     const PRIPPS_CONST Atom<float> *ca = as_const(atoms);
 #pragma unroll
     for (int k = 0; k < K; ++k)
@@ -198,7 +234,7 @@ __device__ __forceinline__ void cheb_chunk(const Atom<float> *__restrict__ atoms
         sqrt_and_rsqrt(r2, r, rinv);
         float take = MASK ? (j > iidx ? 1.0f : 0.0f) : lane_take;
         if constexpr (BYW) take *= to_vgpr(aj.w);
-        cheb_pair<P, K>(r, take * rinv, qc_hi, qc_lo, hh_hi, hh_lo, acc);
+        cheb_pair<P, K>(r, take * rinv, qc_hi, qc_lo, hh_hi, hh_lo, lim, acc);
     };
     // Two pairs per iteration, each atom's scalar load issued one pair ahead
     // into its own registers.  The empty asm that reads an atom at the end of
@@ -273,6 +309,16 @@ __device__ __forceinline__ void cheb_record(PRIPPS_LDS double *d, double *record
         }
 }
 
+// cls_wet[c] through the 32-bit word that holds it: the scalar memory path
+// has no byte loads, and a vector byte load here would make the chunk's carry
+// wait for every vector memory operation still in flight (record stores, the
+// queue fetch).
+static_assert(offsetof(DevStruct, cls_wet) % 4 == 0, "cls_wet is read by words");
+__device__ __forceinline__ bool class_wet(const PRIPPS_CONST DevStruct &sd, int c) {
+    const PRIPPS_CONST int32_t *w = (const PRIPPS_CONST int32_t *)sd.cls_wet;
+    return ((w[c >> 2] >> (8 * (c & 3))) & 0xff) != 0;
+}
+
 template <int P, int K, int RS>
 __device__ __forceinline__ void cheb_item(const TiledArgs &args, const Item &it, PRIPPS_LDS double *d,
                                           int &rec) {
@@ -298,6 +344,7 @@ __device__ __forceinline__ void cheb_item(const TiledArgs &args, const Item &it,
     const double hh = 0.5 * sd.strip_geo[2 * it.strip + 1];
     const float hh_hi = to_vgpr(float(hh));
     const float hh_lo = to_vgpr(float(hh - double(float(hh))));
+    const ChebLimits lim = args.cheb;
 
     int j = it.j0;
     int c = 0;
@@ -307,7 +354,7 @@ __device__ __forceinline__ void cheb_item(const TiledArgs &args, const Item &it,
         if (cur >= 0 && c != cur) cheb_record<RS>(d, args.partials, rec++);
         cur = c;
         const int jend = min(min(j + sd.block, it.j1), sd.cls_start[c + 1]);
-        const bool wet = sd.cls_wet[c] != 0;
+        const bool wet = class_wet(sd, c);
         const bool mask = j <= it.i0 + it.icount - 1;  // chunk meets the diagonal
         // pass 0: sum s (and w_i s); pass 1 (hydrated j only): sum w_j s (and w_i w_j s)
 #pragma unroll
@@ -315,11 +362,11 @@ __device__ __forceinline__ void cheb_item(const TiledArgs &args, const Item &it,
             if (pass == 1 && !wet) break;
             float acc[K][P];
             if (pass == 0) {
-                if (mask) cheb_chunk<P, K, true, false>(atoms, j, jend, ai, iidx, lane_take, qc_hi, qc_lo, hh_hi, hh_lo, acc);
-                else cheb_chunk<P, K, false, false>(atoms, j, jend, ai, iidx, lane_take, qc_hi, qc_lo, hh_hi, hh_lo, acc);
+                if (mask) cheb_chunk<P, K, true, false>(atoms, j, jend, ai, iidx, lane_take, qc_hi, qc_lo, hh_hi, hh_lo, lim, acc);
+                else cheb_chunk<P, K, false, false>(atoms, j, jend, ai, iidx, lane_take, qc_hi, qc_lo, hh_hi, hh_lo, lim, acc);
             } else {
-                if (mask) cheb_chunk<P, K, true, true>(atoms, j, jend, ai, iidx, lane_take, qc_hi, qc_lo, hh_hi, hh_lo, acc);
-                else cheb_chunk<P, K, false, true>(atoms, j, jend, ai, iidx, lane_take, qc_hi, qc_lo, hh_hi, hh_lo, acc);
+                if (mask) cheb_chunk<P, K, true, true>(atoms, j, jend, ai, iidx, lane_take, qc_hi, qc_lo, hh_hi, hh_lo, lim, acc);
+                else cheb_chunk<P, K, false, true>(atoms, j, jend, ai, iidx, lane_take, qc_hi, qc_lo, hh_hi, hh_lo, lim, acc);
             }
             cheb_carry<P, K, RS>(acc, d, 2 * pass, wet_i, wi);
         }
@@ -341,11 +388,14 @@ __global__ void __launch_bounds__(256) debye_cheb_kernel(TiledArgs args) {
     for (int s = 0; s < kChebSets; ++s)
 #pragma unroll
         for (int r = 0; r < RS; r += 64) d[s * RS + r + lane] = 0.0;
-    for (;;) {
-        int w = 0;
-        if (lane == 0) w = atomicAdd(args.counter, 1);
-        w = __builtin_amdgcn_readfirstlane(__shfl(w, 0, 64));
-        if (w >= args.n_items) break;
+    // Unit w of the queue: wave g of the grid takes unit g first, without
+    // touching the queue (at the start every wave would otherwise wait on one
+    // contended atomic), and then units n_waves + counter++ as it finishes.
+    // Waves of one SIMD do not progress alike (the oldest wave issues first),
+    // so the later units go to whichever wave is free.
+    const int n_waves = gridDim.x * 4;
+    int w = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+    while (w < args.n_items) {
         const PRIPPS_CONST Unit &un = as_const(args.units)[w];
         const int i0 = un.item0, i1 = un.item1, nk = un.nk;
         int rec = un.rec0;
@@ -354,6 +404,15 @@ __global__ void __launch_bounds__(256) debye_cheb_kernel(TiledArgs args) {
             if (nk == 2) cheb_item<P, 2, RS>(args, it, d, rec);
             else cheb_item<P, 1, RS>(args, it, d, rec);
         }
+        int nx = 0;
+        if (lane == 0) nx = atomicAdd(args.counter, 1);
+        w = n_waves + __builtin_amdgcn_readfirstlane(__shfl(nx, 0, 64));
+    }
+    // The last wave out leaves the queue at zero for the next launch: every
+    // other wave has taken its last unit index before it counts itself out.
+    if (lane == 0 && atomicAdd(args.counter + 1, 1) == n_waves - 1) {
+        atomicExch(args.counter, 0);
+        atomicExch(args.counter + 1, 0);
     }
 }
 
@@ -405,7 +464,7 @@ __global__ void __launch_bounds__(kChebQ) cheb_combo_kernel(ChebReduceArgs a) {
                 b2 = b1;
                 b1 = b0;
             }
-            s[set] = __builtin_fma(x, b1, c[0] - b2);
+            s[set] = __builtin_fma(x, b1, 0.5 * c[0] - b2);  // c_0 holds w_0 = 2 u_0
         }
         const int nq = sd.nq, ik = sa.w, jk = sb.x;
         const double ak = sd.ffa[ik * nq + m], al = sd.ffa[jk * nq + m];
@@ -465,15 +524,89 @@ hipError_t launch_cheb_reduce(const ChebReduceArgs &a, hipStream_t st) {
     return hipSuccess;
 }
 
+// Largest |w_p| over the recurrence started at order L from (w_{L+1}, w_L) =
+// (0, 1) at beta (FP64; infinity when it leaves the FP64 range).
+static double cheb_ratio(double beta, int L) {
+    const double z = 0.25 * beta * beta;
+    double w1 = 0.0, w0 = 1.0, mx = 1.0;
+    for (int p = L; p >= 1; --p) {
+        const double wm = (p % 2 == 0) ? (-p / z) * w0 + w1 : p * w0 + w1;
+        w1 = w0;
+        w0 = wm;
+        mx = std::max(mx, std::fabs(wm));
+        if (!std::isfinite(mx)) return INFINITY;
+    }
+    return mx;
+}
+
+// The smallest beta whose recurrence from order L stays within 2^bits of its
+// start (the ratio falls as beta grows).
+static double cheb_beta_min(int L, int bits) {
+    double lo = 1e-30, hi = 64.0;
+    for (int it = 0; it < 200; ++it) {
+        const double mid = std::sqrt(lo * hi);
+        if (cheb_ratio(mid, L) <= std::ldexp(1.0, bits)) hi = mid;
+        else lo = mid;
+    }
+    return hi;
+}
+
+// Start values of the recurrence for order P (see ChebLimits).  Every |w_p|
+// stays within [2^-126, 2^50], so that the normalised weights amp sin a / S
+// stay normal FP32 numbers: the top start 2^-113 may grow by 2^163 and takes
+// over from z >= z1 (its threshold, reached within 2^-20 of z1); the
+// restarts 2^-126 may grow by 2^176.  A restart below a start must give the
+// pair's weights for every beta under that start's threshold: the truncation
+// J_L(beta) <= (beta/2)^L / L! stays below 1e-10 there.  z_floor keeps the
+// coefficients -p / z finite and the lowest restart within range; it changes
+// no weight (J_p(beta) = delta_p0 to below 1e-24 there).
+bool cheb_limits(int P, ChebLimits *out) {
+    static std::mutex mu;
+    static ChebLimits cache[64];
+    static bool have[64] = {};
+    if (P <= 0 || P >= 62) return false;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!have[P]) {
+        const int N = P + 2;
+        const bool three = N > 16;
+        auto weight_bound = [](double beta, int L) {
+            double t = 1.0;
+            for (int k = 1; k <= L; ++k) t *= 0.5 * beta / k;
+            return t;
+        };
+        const double t1 = cheb_beta_min(N, 163) * (1.0 + 1e-6);
+        const double t2 = three ? cheb_beta_min(kChebMid, 176) * (1.0 + 1e-6) : t1;
+        if (three && weight_bound(t1 * (1.0 + 1e-5), kChebMid) > 1e-10) return false;
+        if (weight_bound(t2 * (1.0 + 1e-5), kChebLow) > 1e-10) return false;
+        const double tf = cheb_beta_min(kChebLow, 176) * (1.0 + 1e-6);
+        const double zf = std::max(0.25 * tf * tf, std::ldexp(2.0 * N, -120));
+        if (0.25 * tf * tf > 1e-24) return false;
+        ChebLimits L{};
+        L.z_floor = (float)zf * (1.0f + 0x1p-20f);
+        L.k1 = (float)(std::ldexp(1.0, -93) / (0.25 * t1 * t1));
+        L.b1 = -0x1p-93f;
+        if (three) {
+            L.k2 = (float)(std::ldexp(1.0, -106) / (0.25 * t2 * t2));
+            L.b2 = -0x1p-106f;
+        }
+        cache[P] = L;
+        have[P] = true;
+    }
+    *out = cache[P];
+    return true;
+}
+
 // Orders the kernel is built for; the host rounds a structure's need up.
 #define PRIPPS_CHEB_ORDERS(X) \
     X(12) X(16) X(20) X(22) X(24) X(26) X(28) X(30) X(32) X(36) X(40)
 
 hipError_t launch_cheb(int P, const TiledArgs &a, int blocks, hipStream_t st) {
+    TiledArgs b = a;
+    if (!cheb_limits(P, &b.cheb)) return hipErrorInvalidValue;
     switch (P) {
 #define PRIPPS_CASE(PP)                                                                     \
     case PP:                                                                                \
-        hipLaunchKernelGGL(debye_cheb_kernel<PP>, dim3(blocks), dim3(256), 0, st, a);       \
+        hipLaunchKernelGGL(debye_cheb_kernel<PP>, dim3(blocks), dim3(256), 0, st, b);       \
         return hipGetLastError();
         PRIPPS_CHEB_ORDERS(PRIPPS_CASE)
 #undef PRIPPS_CASE

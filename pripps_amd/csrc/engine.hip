@@ -64,7 +64,7 @@ static int hip_fail(hipError_t e, const char *what) {
     } while (0)
 
 extern "C" const char *pripps_last_error(void) { return g_last_error.c_str(); }
-extern "C" const char *pripps_version(void) { return "pripps_amd 0.1.0 (gfx950)"; }
+extern "C" const char *pripps_version(void) { return "pripps_amd 0.2.0 (gfx950)"; }
 
 // ---------------------------------------------------------------------------
 // device block pool for model buffers
@@ -156,9 +156,12 @@ struct LaunchRecord {
     ChebReduceArgs cargs{};   // Chebyshev kernel
 };
 
-// fill counter, pair kernel (bracketed by ev_a/ev_b when given), reduction
+// fill counter (recurrence kernels), pair kernel (bracketed by ev_a/ev_b when
+// given), reduction
 hipError_t run_launch(const LaunchRecord &r, hipStream_t st, hipEvent_t ev_a, hipEvent_t ev_b) {
-    hipError_t err = launch_fill_int(r.counter, 1, 0, st);
+    // the Chebyshev kernel leaves its queue counters at zero when it ends, and
+    // the upload zeroes them, so only the recurrence kernels need the fill
+    hipError_t err = r.Q == kChebQ ? hipSuccess : launch_fill_int(r.counter, 1, 0, st);
     if (err == hipSuccess && ev_a) err = hipEventRecord(ev_a, st);
     if (err == hipSuccess && r.n_items > 0)
         err = r.Q == kChebQ ? launch_cheb(r.P, r.targs, r.blocks, st)
@@ -1288,6 +1291,7 @@ static void stage_build(const BuildPlan &B, const std::vector<Prepared> &P, char
         if (bytes) std::memcpy(hbase + off, src, bytes);
     };
     for (auto &g : B.groups) {
+        std::memset(hbase + g.o_counter, 0, 256);  // queue counters start at zero
         put(g.o_items, g.items.data(), g.items.size() * sizeof(Item));
         put(g.o_units, g.units.data(), g.units.size() * sizeof(Unit));
         put(g.o_chunk_recs, g.chunk_recs.data(), g.chunk_recs.size() * 4);

@@ -107,15 +107,24 @@ struct ChebReduceArgs {
     int n_st;
 };
 
+// Start values of the Chebyshev kernel's Bessel recurrence for one order P
+// (host-computed, cheb_limits): z = (beta / 2)^2 is clamped to z_floor, the
+// start at order P + 2 is clamp(fma(z, k1, b1), 0, 2^-113) and, for P > 14,
+// the restart at order 14 is clamp(fma(z, k2, b2), 0, 2^-126).
+struct ChebLimits {
+    float z_floor, k1, b1, k2, b2;
+};
+
 // Kernel argument blocks.
 struct TiledArgs {
     const DevStruct *structs;
     const Item *items;
     int64_t n_items;   // queue length: items, or units (Chebyshev plans)
-    int *counter;      // work-queue head, zeroed before launch
+    int *counter;      // work-queue head (and, Chebyshev plans, exit count): zero at launch
     double *partials;  // [slots][6][Q]; a slot per item, or per unit and strip
     int variant;       // FP32 uniform inner-loop variant (see debye_tiled_kernel)
     const Unit *units; // Chebyshev plans, else nullptr
+    ChebLimits cheb;   // Chebyshev plans: set by launch_cheb for the launch's order
 };
 
 struct FinalizeArgs {
@@ -156,6 +165,8 @@ hipError_t launch_finalize(const FinalizeArgs &a, int n_strips_total, hipStream_
 // Chebyshev pair kernel (debye_cheb.hip), P = expansion order (a built order)
 hipError_t launch_cheb(int P, const TiledArgs &a, int blocks, hipStream_t st);
 hipError_t cheb_occupancy(int P, int *blocks_per_cu);
+// start values of the pair recurrence for order P; false if P is not supported
+bool cheb_limits(int P, ChebLimits *out);
 hipError_t launch_cheb_reduce(const ChebReduceArgs &a, hipStream_t st);
 // smallest built order whose truncation is below 1e-8 of a pair's amplitude
 // at beta = H r up to beta_max (0: none)
