@@ -337,11 +337,11 @@ __device__ __forceinline__ void cheb_item(const TiledArgs &args, const Item &it,
     float qc_hi[K], qc_lo[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-        const double qcr = sd.strip_geo[2 * (it.strip + k)] * kInv2Pi;
+        const double qcr = as_const(sd.strip_geo)[2 * (it.strip + k)] * kInv2Pi;
         qc_hi[k] = to_vgpr(float(qcr));
         qc_lo[k] = to_vgpr(float(qcr - double(float(qcr))));
     }
-    const double hh = 0.5 * sd.strip_geo[2 * it.strip + 1];
+    const double hh = 0.5 * as_const(sd.strip_geo)[2 * it.strip + 1];
     const float hh_hi = to_vgpr(float(hh));
     const float hh_lo = to_vgpr(float(hh - double(float(hh))));
     const ChebLimits lim = args.cheb;
@@ -404,6 +404,9 @@ __global__ void __launch_bounds__(256) debye_cheb_kernel(TiledArgs args) {
             if (nk == 2) cheb_item<P, 2, RS>(args, it, d, rec);
             else cheb_item<P, 1, RS>(args, it, d, rec);
         }
+        // fetched here, not ahead: at the start every wave would contend for
+        // the counter at once, and every later vector-memory wait (record
+        // stores, the next i tile) would wait for the atomic too
         int nx = 0;
         if (lane == 0) nx = atomicAdd(args.counter, 1);
         w = n_waves + __builtin_amdgcn_readfirstlane(__shfl(nx, 0, 64));
@@ -425,12 +428,34 @@ __global__ void __launch_bounds__(256) cheb_chunk_kernel(ChebReduceArgs a) {
     const int c = blockIdx.x;
     const int b = a.chunk_first[c], e = a.chunk_first[c + 1];
     const int n = kChebSets * a.RS;
+    static_assert(kFinChunk == 32, "one load per record of a chunk in flight");
     for (int t = threadIdx.x; t < n; t += 256) {
+        // every load first, then the sum in record order
+        double v[kFinChunk];
+#pragma unroll
+        for (int i = 0; i < kFinChunk; ++i)
+            v[i] = b + i < e ? a.records[(int64_t)a.chunk_recs[b + i] * n + t] : 0.0;
         double acc = 0.0;
-#pragma unroll 4
-        for (int i = b; i < e; ++i) acc += a.records[(int64_t)a.chunk_recs[i] * n + t];
+#pragma unroll
+        for (int i = 0; i < kFinChunk; ++i)
+            if (b + i < e) acc += v[i];
         a.chunk_tot[(int64_t)c * n + t] = acc;
     }
+}
+
+// sum_{c in [c0, c1)} x[c * stride] in order of c, eight loads in flight
+__device__ __forceinline__ double sum_strided(const double *x, int c0, int c1, int64_t stride) {
+    double acc = 0.0;
+    int c = c0;
+    for (; c + 8 <= c1; c += 8) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = x[(int64_t)(c + k) * stride];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc += v[k];
+    }
+    for (; c < c1; ++c) acc += x[(int64_t)c * stride];
+    return acc;
 }
 
 // Stage 2: block b sums its combo's chunk totals (in order) into LDS, then
@@ -441,20 +466,30 @@ __global__ void __launch_bounds__(kChebQ) cheb_combo_kernel(ChebReduceArgs a) {
     __shared__ double coef[kChebSets * 128];
     const int4 sa = a.sb_a[blockIdx.x], sb = a.sb_b[blockIdx.x];
     const int n = kChebSets * a.RS;
-    for (int t = threadIdx.x; t < n; t += kChebQ) {
-        double acc = 0.0;
-        for (int c = sb.y; c < sb.z; ++c) acc += a.chunk_tot[(int64_t)c * n + t];
-        coef[t] = acc;
-    }
-    __syncthreads();
     const PRIPPS_CONST DevStruct &sd = as_const(a.structs)[sa.x];
     const int strip = sa.y, ql = threadIdx.x;
     const int first = sd.strip_first[strip], end = sd.strip_first[strip + 1];
     const int m = first + ql;
+    // this q-point's grid value and form factors, loaded while the block sums
+    const int nq = sd.nq, ik = sa.w, jk = sb.x;
+    double qm = 0.0, ak = 0.0, al = 0.0, ek = 0.0, el = 0.0, wq = 0.0;
+    if (m < end) {
+        qm = sd.q[m];
+        ak = sd.ffa[ik * nq + m];
+        al = sd.ffa[jk * nq + m];
+        if (sd.ffe) {
+            ek = sd.ffe[ik * nq + m];
+            el = sd.ffe[jk * nq + m];
+        }
+        if (sd.ffw) wq = sd.ffw[m];
+    }
+    for (int t = threadIdx.x; t < n; t += kChebQ)
+        coef[t] = sum_strided(a.chunk_tot + t, sb.y, sb.z, n);
+    __syncthreads();
     double v[kProfiles] = {0, 0, 0, 0, 0, 0};
     if (m < end) {
         const double qc = sd.strip_geo[2 * strip], H = sd.strip_geo[2 * strip + 1];
-        const double x = H > 0.0 ? (sd.q[m] - qc) / H : 0.0;
+        const double x = H > 0.0 ? (qm - qc) / H : 0.0;
         double s[kChebSets];
         for (int set = 0; set < kChebSets; ++set) {
             const double *c = coef + set * a.RS + sa.z * a.P;
@@ -466,11 +501,6 @@ __global__ void __launch_bounds__(kChebQ) cheb_combo_kernel(ChebReduceArgs a) {
             }
             s[set] = __builtin_fma(x, b1, 0.5 * c[0] - b2);  // c_0 holds w_0 = 2 u_0
         }
-        const int nq = sd.nq, ik = sa.w, jk = sb.x;
-        const double ak = sd.ffa[ik * nq + m], al = sd.ffa[jk * nq + m];
-        const double ek = sd.ffe ? sd.ffe[ik * nq + m] : 0.0;
-        const double el = sd.ffe ? sd.ffe[jk * nq + m] : 0.0;
-        const double wq = sd.ffw ? sd.ffw[m] : 0.0;
         // sets: S = sum s, Si = sum w_i s, Sw = sum w_j s, Siw = sum w_i w_j s
         v[0] = 2.0 * ak * al * s[0];
         v[1] = 2.0 * ek * el * s[0];
@@ -492,12 +522,10 @@ __global__ void __launch_bounds__(kChebQ) cheb_strip_kernel(ChebReduceArgs a) {
     const int strip = st.y, ql = threadIdx.x;
     const int m = sd.strip_first[strip] + ql;
     if (m >= sd.strip_first[strip + 1]) return;
-    double acc[kProfiles] = {0, 0, 0, 0, 0, 0};
-    for (int b = st.z; b < st.w; ++b) {
-        const double *p = a.sb_out + (int64_t)b * (kProfiles * kChebQ) + ql;
+    double acc[kProfiles];
 #pragma unroll
-        for (int pr = 0; pr < kProfiles; ++pr) acc[pr] += p[pr * kChebQ];
-    }
+    for (int pr = 0; pr < kProfiles; ++pr)
+        acc[pr] = sum_strided(a.sb_out + pr * kChebQ + ql, st.z, st.w, kProfiles * kChebQ);
     double diag[kProfiles], zero[kProfiles];
     diagonal_and_zero(sd, m, diag, zero);
     const double q = sd.q[m];
