@@ -204,11 +204,16 @@ int pripps_debye_model_stats(const pripps_debye_model *model, pripps_build_stats
 
 /*
  * Re-run the pair and finalize kernels of the engine's most recent build
- * `repeats` times on its device-resident inputs (no host work, no upload) and
- * report the mean device time of the pair kernel per run (HIP events on the
- * engine stream).  `model` must belong to that build (any model of a batch);
- * otherwise PRIPPS_ERR_USAGE.  Results are bitwise those of the build.  No
- * reference counterpart: it exists to measure the kernel with inputs in HBM.
+ * `repeats` times on its device-resident inputs (no host work, no upload).
+ * With `pair_ms_mean` non-NULL the kernels are launched one by one and
+ * `*pair_ms_mean` is the mean device time of the pair kernel per run (HIP
+ * events on the engine stream).  With `pair_ms_mean` NULL each run is one
+ * launch of a HIP graph that holds the build's kernel launches (captured at
+ * the first such call after a build, kept until the next build).
+ * `*total_ms` (if non-NULL): device time of all runs.  `model` must belong to
+ * that build (any model of a batch); otherwise PRIPPS_ERR_USAGE.  Results
+ * are bitwise those of the build.  No reference counterpart: it exists to
+ * measure the kernels and replay them with inputs in HBM.
  */
 int pripps_debye_model_recompute(pripps_debye_model *model, int32_t repeats,
                                  double *pair_ms_mean, double *total_ms);

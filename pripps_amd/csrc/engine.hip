@@ -194,6 +194,11 @@ struct pripps_engine {
     uint64_t build_serial = 0;
     bool last_plain = false;
     std::vector<LaunchRecord> last_launches;
+    // those launches captured as one HIP graph (replays without per-kernel
+    // timing), for the build `replay_serial`
+    hipGraph_t replay_graph = nullptr;
+    hipGraphExec_t replay_exec = nullptr;
+    uint64_t replay_serial = 0;
     // autotuned plans per size class (pripps_engine_autotune)
     std::mutex tune_mu;
     std::vector<pripps_tuning_entry> tune_table;
@@ -333,6 +338,8 @@ static void engine_release(pripps_engine *e) {
     if (e->fit_buf) (void)hipFree(e->fit_buf);
     if (e->arena) (void)hipFree(e->arena);
     if (e->staging) (void)hipHostFree(e->staging);
+    if (e->replay_exec) (void)hipGraphExecDestroy(e->replay_exec);
+    if (e->replay_graph) (void)hipGraphDestroy(e->replay_graph);
     if (e->ev0) (void)hipEventDestroy(e->ev0);
     if (e->ev1) (void)hipEventDestroy(e->ev1);
     if (e->stream) (void)hipStreamDestroy(e->stream);
@@ -1827,11 +1834,57 @@ extern "C" int pripps_debye_model_fit_curve(const pripps_debye_model *m, double 
     return PRIPPS_OK;
 }
 
+// The engine's most recent launches as one instantiated HIP graph, captured at
+// the first graph replay of a build and kept until the next build.  The
+// launches hold kernels only (no memset nodes): the Chebyshev kernel leaves
+// its queue counters at zero and the recurrence kernels' counter is zeroed
+// by a fill kernel.  The caller holds e->mu with e's device current.
+static int ensure_replay_graph(pripps_engine *e) {
+    if (e->replay_exec && e->replay_serial == e->build_serial) return PRIPPS_OK;
+    if (e->replay_exec) (void)hipGraphExecDestroy(e->replay_exec);
+    if (e->replay_graph) (void)hipGraphDestroy(e->replay_graph);
+    e->replay_exec = nullptr;
+    e->replay_graph = nullptr;
+    hipStream_t st = e->stream;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    hipError_t err = hipSuccess;
+    for (const auto &r : e->last_launches)
+        if (err == hipSuccess) err = run_launch(r, st, nullptr, nullptr);
+    hipGraph_t g = nullptr;
+    const hipError_t eerr = hipStreamEndCapture(st, &g);
+    if (err == hipSuccess) err = eerr;
+    if (err == hipSuccess) err = hipGraphInstantiate(&e->replay_exec, g, nullptr, nullptr, 0);
+    if (err != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        e->replay_exec = nullptr;
+        return hip_fail(err, "capture of the replay graph");
+    }
+    e->replay_graph = g;
+    e->replay_serial = e->build_serial;
+    return PRIPPS_OK;
+}
+
 // Replay the engine's most recent launches `repeats` times; the caller holds
-// e->mu.  pair_ms: mean device time of the pair kernels per replay.
+// e->mu.  With pair_ms_mean: eager launches with HIP events around each pair
+// kernel, pair_ms_mean = their mean device time per replay.  Without: one
+// launch of the captured graph per replay.  total_ms: device time of all
+// replays.
 static int replay_timed(pripps_engine *e, int32_t repeats, double *pair_ms_mean, double *total_ms) {
     HIPCHK(hipSetDevice(e->device));
     hipStream_t st = e->stream;
+    if (!pair_ms_mean) {
+        int rc = ensure_replay_graph(e);
+        if (rc) return rc;
+        hipError_t err = hipEventRecord(e->ev0, st);
+        for (int r = 0; r < repeats && err == hipSuccess; ++r) err = hipGraphLaunch(e->replay_exec, st);
+        if (err == hipSuccess) err = hipEventRecord(e->ev1, st);
+        if (err == hipSuccess) err = hipStreamSynchronize(st);
+        if (err != hipSuccess) return hip_fail(err, "graph replay");
+        float tot = 0;
+        (void)hipEventElapsedTime(&tot, e->ev0, e->ev1);
+        if (total_ms) *total_ms = tot;
+        return PRIPPS_OK;
+    }
     const size_t nl = e->last_launches.size();
     std::vector<hipEvent_t> ev(2 * (size_t)repeats * nl + 2, nullptr);
     for (auto &x : ev) HIPCHK(hipEventCreate(&x));

@@ -186,13 +186,16 @@ class NativeModel:
                                                       N.dptr(out)))
         return out
 
-    def recompute(self, repeats: int):
+    def recompute(self, repeats: int, graph: bool = False):
         """Re-run the last build's kernels on resident inputs; returns
-        (mean pair-kernel ms, total device ms)."""
+        (mean pair-kernel ms, total device ms).  graph=True replays them as
+        one captured HIP graph launch per run (the pair-kernel time is then
+        not measured and returned as None)."""
         a, b = ctypes.c_double(), ctypes.c_double()
-        N.check(N.load().pripps_debye_model_recompute(self._h, int(repeats), ctypes.byref(a),
+        N.check(N.load().pripps_debye_model_recompute(self._h, int(repeats),
+                                                      None if graph else ctypes.byref(a),
                                                       ctypes.byref(b)))
-        return a.value, b.value
+        return (None if graph else a.value), b.value
 
     def stats(self) -> N.BuildStats:
         s = N.BuildStats()

@@ -93,3 +93,36 @@ def test_cheb_batch_equals_single(engine):
     batch = engine.build_batch([PreparedInput(*c) for c in cases], "fp32", t)
     for c, bm in zip(cases, batch):
         assert np.array_equal(build(engine, c, "fp32", t).profiles(), bm.profiles())
+
+
+@pytest.mark.parametrize("precision,tune", [("fp32", None), ("fp64", None), ("fp32", "q64")])
+def test_graph_replays_match_build(engine, precision, tune):
+    """recompute(graph=True) replays the build's kernel launches as one
+    captured HIP graph per run.  Every replay -- with eager replays,
+    intensity() calls and host synchronisations in between -- reproduces the
+    build's profiles bit for bit; the Chebyshev plan (counters reset by the
+    kernel), the FP64 recurrence plan and an FP32 recurrence plan (counter
+    zeroed by a fill kernel inside the graph) alike.  A later build is
+    replayed from a new capture, and the earlier model can no longer be
+    replayed."""
+    from pripps_amd._native import tuning
+    from pripps_amd.errors import UsageError
+    t = tuning(q_per_lane=64) if tune == "q64" else None
+    case = make_case(1800, 41, nq=300)
+    m = build(engine, case, precision, t)
+    ref = m.profiles()
+    for k in range(3):
+        _, total = m.recompute(2 + k, graph=True)
+        assert total > 0
+        assert np.array_equal(m.profiles(), ref), k
+        m.recompute(1)
+        m.intensity(1.0, 0.0)
+        assert np.array_equal(m.profiles(), ref), k
+    other = make_case(900, 42, nq=120)
+    m2 = build(engine, other, precision, t)
+    ref2 = m2.profiles()
+    m2.recompute(3, graph=True)
+    assert np.array_equal(m2.profiles(), ref2)
+    assert np.array_equal(m.profiles(), ref)  # the earlier model's profiles stay
+    with pytest.raises(UsageError):
+        m.recompute(1, graph=True)
