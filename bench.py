@@ -58,8 +58,18 @@ COUNTERS = {
         "valu_wave_instr": 1.7960e8, "cycles_per_xcd": 5.1347e6 / 8,
         "profile": "profiles/debye_cheb_f32_solvent10k.md"},
 }
-F64_EXECUTED_FLOP_PER_TERM = 6.71
-F64_VALU_WAVE_INSTR = 2.299e9
+# The same figures for the FP64 pair kernel on the default FP64 plan of
+# solvent10k (profiles/debye_cheb64_f64_solvent10k.md); executed FLOP from
+# SQ_INSTS_VALU_{FMA,ADD,MUL,TRANS}_F64.  FP64 instructions do not dual-issue:
+# one per quad-cycle per SIMD at most.
+COUNTERS_F64 = {
+    ("pripps_amd 0.2.0 (gfx950)", "debye_cheb64_kernel<36>", 256, 2048): {
+        "bytes": (2231.9 * 2.0 + 7100.1 * 1.0) * 1024,
+        "executed_flop_per_term": 64.0 * (2 * 1.9034e8 + 4.1058e7 + 5.9006e7 + 5.1888e6) / 2.49975e10,
+        "valu_wave_instr": 3.4242e8, "cycles_per_xcd": 1.3558e7 / 8,
+        "profile": "profiles/debye_cheb64_f64_solvent10k.md"},
+}
+VALU_ISSUE_PEAK_F64 = 0.25
 # VALU wave instructions a SIMD can issue per cycle (two per quad-cycle with
 # dual issue), and the engine clock
 VALU_ISSUE_PEAK = 0.5
@@ -95,6 +105,9 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--span", type=int, default=0)
     ap.add_argument("--variant", type=int, default=0)
+    ap.add_argument("--fp64-q-per-lane", type=int, default=0,
+                    help="q-points per strip of the FP64 leg (0: the default plan; 32 or 64: "
+                         "the tiled FP64 recurrence kernel)")
     ap.add_argument("--cpu-q", type=int, default=8, help="q points of the CPU baseline sample")
     ap.add_argument("--cpu-q1", type=int, default=1,
                     help="q points of the one-core CPU baseline sample (0: skip)")
@@ -165,24 +178,38 @@ def main():
         s = m64.stats()
         extra["fp64_plain_kernel_ms"] = s.kernel_ms
         log(f"fp64_plain build: kernel {s.kernel_ms:.2f} ms")
-        m64t = eng.build(inp, "fp64")
+        m64t = eng.build(inp, "fp64", _native.tuning(args.fp64_q_per_lane) if args.fp64_q_per_lane
+                         else None)
         fp64_prof = m64t.profiles()
+        s64 = m64t.stats()
         m64t.recompute(1)
         pm, _ = m64t.recompute(5)
-        extra["fp64_tiled_pair_kernel_ms"] = pm
-        # executed FP64 FLOP per term and VALU instructions per dispatch of the
-        # Q 64 plan from its counters (profiles/debye_tiled_f64_q64_solvent10k.md);
-        # FP64 instructions issue at most one per quad-cycle
-        tf64 = F64_EXECUTED_FLOP_PER_TERM * terms / (pm * 1e-3) / 1e12
-        extra["fp64_roofline"] = {
-            "bound": "valu", "achieved": tf64, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": tf64 / FP64_PEAK_TFLOPS, "q_per_lane": m64t.stats().q_per_lane,
-            "executed_flop_per_term": F64_EXECUTED_FLOP_PER_TERM,
-            "issue_frac": F64_VALU_WAVE_INSTR / (1024 * pm * 1e-3 * CLOCK_HZ) / 0.25,
-            "achieved_algorithmic": FLOP_PER_TERM_SOLVENT * terms / (pm * 1e-3) / 1e12,
-            "flop_per_term_algorithmic": FLOP_PER_TERM_SOLVENT}
-        extra["fp64_tiled_vs_plain_maxerr"] = envelope_error(fp64_prof, plain_prof)
-        log(f"fp64 tiled pair kernel {pm:.2f} ms; tiled vs plain {extra['fp64_tiled_vs_plain_maxerr']:.2e}")
+        k64 = (f"debye_cheb64_kernel<{s64.cheb_order}>" if s64.q_per_lane == 256 else
+               "debye_tiled_f64_kernel<%d>" % s64.q_per_lane)
+        log(f"fp64 build: Q={s64.q_per_lane} P={s64.cheb_order} units={s64.work_items} "
+            f"kernel {k64}")
+        extra["fp64_pair_kernel_ms"] = pm
+        extra["fp64_kernel"] = k64
+        extra["fp64_vs_plain_pair_kernel"] = s.kernel_ms / pm
+        ctr64 = COUNTERS_F64.get((_native.load().pripps_version().decode(), k64, s64.q_per_lane,
+                                  s64.work_items))
+        r64 = {"bound": "valu", "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "kernel": k64,
+               "kernel_ms": pm, "achieved": None, "frac": None, "issue_frac": None,
+               "achieved_algorithmic": FLOP_PER_TERM_SOLVENT * terms / (pm * 1e-3) / 1e12,
+               "flop_per_term_algorithmic": FLOP_PER_TERM_SOLVENT}
+        if ctr64 is not None:
+            tf64 = ctr64["executed_flop_per_term"] * terms / (pm * 1e-3) / 1e12
+            r64.update({
+                "achieved": tf64, "frac": tf64 / FP64_PEAK_TFLOPS,
+                "executed_flop_per_term": ctr64["executed_flop_per_term"],
+                "traffic": ctr64["bytes"],
+                "issue_frac": ctr64["valu_wave_instr"] / (1024 * pm * 1e-3 * CLOCK_HZ) / VALU_ISSUE_PEAK_F64,
+                "issue_frac_profiled": ctr64["valu_wave_instr"] / 1024 / ctr64["cycles_per_xcd"]
+                / VALU_ISSUE_PEAK_F64,
+                "source": ctr64["profile"]})
+        extra["fp64_roofline"] = r64
+        extra["fp64_vs_plain_maxerr"] = envelope_error(fp64_prof, plain_prof)
+        log(f"fp64 pair kernel {pm:.3f} ms; against the plain path {extra['fp64_vs_plain_maxerr']:.2e}")
         del m64, m64t
 
     model = eng.build(inp, "fp32", tune)

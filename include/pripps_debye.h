@@ -233,9 +233,9 @@ int pripps_debye_model_free(pripps_debye_model *model);
  * plan whose work lists do not fit is built with longer j-spans (fewer items
  * or units) instead, which changes the FP32 summation order but not the
  * precision bound; a structure that does not fit even so is refused with
- * PRIPPS_ERR_OUT_OF_MEMORY (FP32 Chebyshev plan on a uniform grid of 500 q:
- * 200,000 globule atoms fit, 300,000 do not; FP64: 55,000 fit, 65,000 do
- * not).  Calls
+ * PRIPPS_ERR_OUT_OF_MEMORY (Chebyshev plans on a uniform grid of 500 q:
+ * FP32 120,000 globule atoms fit, 125,000 do not; FP64 110,000 fit, 120,000
+ * do not).  Calls
  * on one GPU share the array and are serialised by the library.  Precision:
  * FP32_COMPENSATED or FP64.
  */

@@ -69,6 +69,7 @@
 
 #include "internal.h"
 #include "pair_math.h"
+#include "trig_f64.h"
 
 namespace pripps {
 
@@ -420,6 +421,220 @@ __global__ void __launch_bounds__(256) debye_cheb_kernel(TiledArgs args) {
 }
 
 // ---------------------------------------------------------------------------
+// FP64 form (FP64-throughout builds)
+// ---------------------------------------------------------------------------
+// The same expansion, strips, groups, units and coefficient records as the
+// FP32 kernel, computed in FP64: orders whose truncation stays below 1e-15 of
+// a pair's amplitude, the weights by the w-recurrence in FP64, and each
+// lane's coefficient sums in FP64 over the whole class run (no FP32 chunks,
+// no carry), summed across the wave by a butterfly when the run ends.
+//
+// Range: FP64 holds the growth N! (2 / beta)^N down to beta ~ 1e-6, so two
+// starts suffice: 2^-600 at order N for beta above t1 (growth from N below
+// 2^900, ChebLimits64), 0 below it, and a restart of 2^-1000 at order 4 for
+// every lane (growth below 2^1000 down to z_floor); J_4(t1) is below 1e-24.
+constexpr double kCheb64StartTop = 0x1p-600;
+constexpr double kCheb64StartLow = 0x1p-1000;
+constexpr int kCheb64Low = 4;
+
+template <int P, int K>
+__device__ __forceinline__ void cheb64_pair(double r, double amp, const double (&qc)[K], double hh,
+                                            const ChebLimits64 &lim, const double (&kodd)[P / 2 + 1],
+                                            const SinCosCoeffs &kc, double (&acc)[K][P]) {
+    constexpr int N = P + 2;
+    double sa[K], ca[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) sincos_rev_f64(qc[k] * r, sa[k], ca[k], kc);
+    const double b = hh * r;  // beta / 2
+    const double z = fmax(b * b, lim.z_floor);
+    double rz = __builtin_amdgcn_rcp(z);
+    rz = fma(rz, fma(-z, rz, 1.0), rz);
+    rz = fma(rz, fma(-z, rz, 1.0), rz);
+    const double s_top = fmin(fmax(fma(z, lim.k1, lim.b1), 0.0), kCheb64StartTop);
+    double w[P];
+    double wn = 0.0, wc = s_top;
+    double E = ((N / 2) % 2 == 0) ? s_top : -s_top;
+#pragma unroll
+    for (int p = N; p >= 1; --p) {
+        double add = wn;
+        if (p - 1 == kCheb64Low) add += kCheb64StartLow;
+        const double wm = (p % 2 == 0) ? fma(double(-p) * rz, wc, add)
+                          : p == 1     ? wc + add
+                                       : fma(kodd[(p - 1) / 2], wc, add);
+        if (p - 1 < P) w[p - 1] = wm;
+        if ((p - 1) % 2 == 0 && p - 1 >= 2) E = ((p - 1) / 2 % 2 == 0) ? E + wm : E - wm;
+        wn = wc;
+        wc = wm;
+    }
+    const double S = fma(0.5, w[0], E);
+    double inv = __builtin_amdgcn_rcp(S);
+    inv = fma(inv, fma(-S, inv, 1.0), inv);
+    inv = fma(inv, fma(-S, inv, 1.0), inv);
+    const double a0 = amp * inv;
+    const double a0b = a0 * b;
+    double e[K], o[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        e[k] = a0 * sa[k];
+        o[k] = a0b * ca[k];
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[k][p] = fma(w[p], (p % 2 == 0) ? e[k] : o[k], acc[k][p]);
+}
+
+// r = sqrt(r2) and 1 / r in FP64 (reciprocal square root estimate, two
+// Newton steps, and the residual-corrected product)
+__device__ __forceinline__ void sqrt_and_rsqrt_f64(double r2, double &r, double &rinv) {
+    double y = __builtin_amdgcn_rsq(r2);
+    y = fma(0.5 * y, fma(-r2 * y, y, 1.0), y);
+    y = fma(0.5 * y, fma(-r2 * y, y, 1.0), y);
+    const double r0 = r2 * y;
+    r = fma(fma(-r0, r0, r2), 0.5 * y, r0);
+    rinv = fma(y, fma(-r, y, 1.0), y);
+}
+
+// The pairs of j in [jb, je) with the lane's i atom, added to acc.  MASK:
+// only j > i counts (the run meets the diagonal).  BYW: amplitudes times w_j.
+template <int P, int K, bool MASK, bool BYW>
+__device__ __forceinline__ void cheb64_span(const Atom<double> *__restrict__ atoms, int jb, int je,
+                                            const Atom<double> &ai, int iidx, double lane_take,
+                                            const double (&qc)[K], double hh, const ChebLimits64 &lim,
+                                            const double (&kodd)[P / 2 + 1], const SinCosCoeffs &kc,
+                                            double (&acc)[K][P]) {
+    const PRIPPS_CONST Atom<double> *ca = as_const(atoms);
+    for (int j = jb; j < je; ++j) {
+        const Atom<double> aj = load_atom(ca, j);
+        const double dx = ai.x - aj.x, dy = ai.y - aj.y, dz = ai.z - aj.z;
+        const double r2 = fmax(fma(dx, dx, fma(dy, dy, dz * dz)), 1e-24);  // r -> 0: the limit s = q
+        double r, rinv;
+        sqrt_and_rsqrt_f64(r2, r, rinv);
+        double take = MASK ? (j > iidx ? 1.0 : 0.0) : lane_take;
+        if constexpr (BYW) take *= aj.w;
+        cheb64_pair<P, K>(r, take * rinv, qc, hh, lim, kodd, kc, acc);
+    }
+}
+
+// acc (times scale) summed over the wave into record set `set`: coefficient
+// f = k P + p of the group at o[set * RS + f] (butterfly over the lanes, in a
+// fixed order; lane f mod 64 writes it).
+template <int P, int K, int RS>
+__device__ __forceinline__ void cheb64_store(const double (&acc)[K][P], double scale, double *o,
+                                             int set) {
+    const int lane = __lane_id();
+    double tot[RS / 64];
+#pragma unroll
+    for (int r = 0; r < RS / 64; ++r) tot[r] = 0.0;
+#pragma unroll
+    for (int f = 0; f < K * P; ++f) {
+        double v = acc[f / P][f % P] * scale;
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+        if (lane == f % 64) tot[f / 64] = v;
+    }
+#pragma unroll
+    for (int r = 0; r < RS / 64; ++r) o[set * RS + r * 64 + lane] = tot[r];
+}
+
+template <int RS>
+__device__ __forceinline__ void cheb64_zero(double *o, int set) {
+#pragma unroll
+    for (int r = 0; r < RS / 64; ++r) o[set * RS + r * 64 + __lane_id()] = 0.0;
+}
+
+template <int P, int K, int RS>
+__device__ __forceinline__ void cheb64_item(const TiledArgs &args, const Item &it, int &rec,
+                                            const double (&kodd)[P / 2 + 1], const SinCosCoeffs &kc) {
+    const int lane = __lane_id();
+    const PRIPPS_CONST DevStruct &sd = as_const(args.structs)[it.s];
+    const Atom<double> *__restrict__ atoms = static_cast<const Atom<double> *>(sd.atoms);
+    const bool ivalid = lane < it.icount;
+    const int iidx = ivalid ? it.i0 + lane : 0x7fffffff;
+    const Atom<double> ai = atoms[ivalid ? it.i0 + lane : it.i0];
+    const double wi = ivalid ? ai.w : 0.0;
+    const double lane_take = ivalid ? 1.0 : 0.0;
+    const bool wet_i = __ballot(wi != 0.0) != 0;
+    double qc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) qc[k] = as_const(sd.strip_geo)[2 * (it.strip + k)] * kInv2Pi;
+    const double hh = 0.5 * as_const(sd.strip_geo)[2 * it.strip + 1];
+    const ChebLimits64 lim = args.cheb64;
+    const int jdiag = it.i0 + it.icount;  // runs below this meet the diagonal
+    int j = it.j0;
+    int c = 0;
+    while (j < it.j1) {
+        while (sd.cls_start[c + 1] <= j) ++c;
+        const int jr = min(it.j1, sd.cls_start[c + 1]);
+        const int jm = max(j, min(jr, jdiag));  // [j, jm) meets the diagonal
+        const bool wet = class_wet(sd, c);
+        double *o = args.partials + (int64_t)rec * (kChebSets * RS);
+        // pass 0: sum s (and w_i s); pass 1 (hydrated j only): sum w_j s (and w_i w_j s)
+#pragma unroll
+        for (int pass = 0; pass < 2; ++pass) {
+            double acc[K][P];
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+#pragma unroll
+                for (int p = 0; p < P; ++p) acc[k][p] = 0.0;
+            if (pass == 1 && !wet) {
+                cheb64_zero<RS>(o, 2);
+                cheb64_zero<RS>(o, 3);
+                break;
+            }
+            if (pass == 0) {
+                cheb64_span<P, K, true, false>(atoms, j, jm, ai, iidx, lane_take, qc, hh, lim, kodd, kc, acc);
+                cheb64_span<P, K, false, false>(atoms, jm, jr, ai, iidx, lane_take, qc, hh, lim, kodd, kc, acc);
+            } else {
+                cheb64_span<P, K, true, true>(atoms, j, jm, ai, iidx, lane_take, qc, hh, lim, kodd, kc, acc);
+                cheb64_span<P, K, false, true>(atoms, jm, jr, ai, iidx, lane_take, qc, hh, lim, kodd, kc, acc);
+            }
+            cheb64_store<P, K, RS>(acc, 1.0, o, 2 * pass);
+            if (wet_i) cheb64_store<P, K, RS>(acc, wi, o, 2 * pass + 1);
+            else cheb64_zero<RS>(o, 2 * pass + 1);
+        }
+        ++rec;
+        j = jr;
+    }
+}
+
+template <int P>
+__global__ void __launch_bounds__(256) debye_cheb64_kernel(TiledArgs args) {
+    constexpr int RS = (P + 63) / 64 * 64;  // one strip per group
+    const int lane = __lane_id();
+    const int n_waves = gridDim.x * 4;
+    // The odd orders of the recurrence and the sine/cosine coefficients as
+    // registers the compiler cannot see through: as 64-bit constants they
+    // only fit two-address FMAs, each after a register copy of the addend.
+    double kodd[P / 2 + 1];
+#pragma unroll
+    for (int m = 0; m <= P / 2; ++m) {
+        kodd[m] = double(2 * m + 1);
+        asm("" : "+s"(kodd[m]));
+    }
+    SinCosCoeffs kc = sincos_coeffs_f64();
+#pragma unroll
+    for (int i = 0; i < 9; ++i) asm("" : "+v"(kc.s[i]));
+#pragma unroll
+    for (int i = 0; i < 8; ++i) asm("" : "+v"(kc.c[i]));
+    int w = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+    while (w < args.n_items) {
+        const PRIPPS_CONST Unit &un = as_const(args.units)[w];
+        const int i0 = un.item0, i1 = un.item1, nk = un.nk;
+        int rec = un.rec0;
+        (void)nk;  // FP64 plans keep one strip per group (see cheb_strips)
+        for (int i = i0; i < i1; ++i) cheb64_item<P, 1, RS>(args, load_item(args.items, i), rec, kodd, kc);
+        int nx = 0;
+        if (lane == 0) nx = atomicAdd(args.counter, 1);
+        w = n_waves + __builtin_amdgcn_readfirstlane(__shfl(nx, 0, 64));
+    }
+    if (lane == 0 && atomicAdd(args.counter + 1, 1) == n_waves - 1) {
+        atomicExch(args.counter, 0);
+        atomicExch(args.counter + 1, 0);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // reduction of the coefficient records
 // ---------------------------------------------------------------------------
 // Stage 1: block c sums the records of chunk c (up to kFinChunk records of one
@@ -622,6 +837,81 @@ bool cheb_limits(int P, ChebLimits *out) {
     }
     *out = cache[P];
     return true;
+}
+
+// FP64 form: start values (see ChebLimits64 and the FP64 kernel's comment).
+bool cheb64_limits(int P, ChebLimits64 *out) {
+    static std::mutex mu;
+    static ChebLimits64 cache[64];
+    static bool have[64] = {};
+    if (P <= 0 || P >= 62) return false;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!have[P]) {
+        const int N = P + 2;
+        const double t1 = cheb_beta_min(N, 900) * (1.0 + 1e-9);
+        double wb = 1.0;  // J_4(t1) bound
+        for (int k = 1; k <= kCheb64Low; ++k) wb *= 0.5 * t1 / k;
+        if (wb > 1e-20) return false;
+        const double tf = cheb_beta_min(kCheb64Low, 1000) * (1.0 + 1e-9);
+        ChebLimits64 L{};
+        L.z_floor = std::max(0.25 * tf * tf, std::ldexp(2.0 * N, -1000));
+        L.k1 = std::ldexp(1.0, -580) / (0.25 * t1 * t1);
+        L.b1 = -std::ldexp(1.0, -580);
+        cache[P] = L;
+        have[P] = true;
+    }
+    *out = cache[P];
+    return true;
+}
+
+static bool cheb64_order_ok(int P, double beta_max) {
+    const double b = 0.5 * beta_max;
+    double term = 1.0;
+    for (int k = 1; k <= P; ++k) term *= b / k;
+    double tail = 0.0, t = term;
+    for (int k = P; k < P + 300 && t > 1e-40; ++k) {
+        tail += t;
+        t *= b / (k + 1);
+    }
+    return 2.0 * tail < 1e-15;
+}
+
+double cheb64_beta_limit(int P) {
+    double lo = 0.0, hi = 64.0;
+    for (int it = 0; it < 60; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (cheb64_order_ok(P, mid)) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+#define PRIPPS_CHEB64_ORDERS(X) X(16) X(20) X(24) X(28) X(32) X(36)
+
+hipError_t launch_cheb64(int P, const TiledArgs &a, int blocks, hipStream_t st) {
+    TiledArgs b = a;
+    if (!cheb64_limits(P, &b.cheb64)) return hipErrorInvalidValue;
+    switch (P) {
+#define PRIPPS_CASE(PP)                                                                     \
+    case PP:                                                                                \
+        hipLaunchKernelGGL(debye_cheb64_kernel<PP>, dim3(blocks), dim3(256), 0, st, b);     \
+        return hipGetLastError();
+        PRIPPS_CHEB64_ORDERS(PRIPPS_CASE)
+#undef PRIPPS_CASE
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t cheb64_occupancy(int P, int *blocks_per_cu) {
+    switch (P) {
+#define PRIPPS_CASE(PP)                                                                     \
+    case PP:                                                                                \
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu,                 \
+                                                            debye_cheb64_kernel<PP>, 256, 0);
+        PRIPPS_CHEB64_ORDERS(PRIPPS_CASE)
+#undef PRIPPS_CASE
+    }
+    return hipErrorInvalidValue;
 }
 
 // Orders the kernel is built for; the host rounds a structure's need up.

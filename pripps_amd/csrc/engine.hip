@@ -164,8 +164,9 @@ hipError_t run_launch(const LaunchRecord &r, hipStream_t st, hipEvent_t ev_a, hi
     hipError_t err = r.Q == kChebQ ? hipSuccess : launch_fill_int(r.counter, 1, 0, st);
     if (err == hipSuccess && ev_a) err = hipEventRecord(ev_a, st);
     if (err == hipSuccess && r.n_items > 0)
-        err = r.Q == kChebQ ? launch_cheb(r.P, r.targs, r.blocks, st)
-                            : launch_tiled(r.fp64, r.Q, r.uniform, r.targs, r.blocks, st);
+        err = r.Q != kChebQ ? launch_tiled(r.fp64, r.Q, r.uniform, r.targs, r.blocks, st)
+              : r.fp64      ? launch_cheb64(r.P, r.targs, r.blocks, st)
+                            : launch_cheb(r.P, r.targs, r.blocks, st);
     if (err == hipSuccess && ev_b) err = hipEventRecord(ev_b, st);
     if (err == hipSuccess && r.Q == kChebQ) err = launch_cheb_reduce(r.cargs, st);
     else if (err == hipSuccess && r.n_strips > 0) err = launch_finalize(r.fargs, r.n_strips, st);
@@ -587,9 +588,9 @@ struct StructPlan {
 };
 
 // Q: q-points per lane strip of the recurrence kernels (32, 64, 128), or
-// kChebQ for the Chebyshev kernel (FP32, any grid).
+// kChebQ for the Chebyshev kernels (FP32 or FP64, any grid).
 int valid_q(int precision, bool uniform, int Q) {
-    if (precision == PRIPPS_FP64) return (Q == 32 || (Q == 64 && uniform)) ? Q : 0;
+    if (precision == PRIPPS_FP64) return (Q == 32 || (Q == 64 && uniform) || Q == kChebQ) ? Q : 0;
     return (Q == 32 || Q == 64 || (Q == 128 && uniform) || Q == kChebQ) ? Q : 0;
 }
 
@@ -603,24 +604,31 @@ int valid_q(int precision, bool uniform, int Q) {
 // pair's Bessel weights depend on beta alone, so one recurrence serves both)
 // where that H still meets both strips' low-q bound.  Each built order is
 // tried with greedy strips and the one with the fewest instructions per pair
-// over the grid wins: about 32 + 4 P per group (geometry, recurrence, the
-// scaled weights) and 22 + P per strip (its sine and cosine, its P sums).
+// over the grid wins: about 30 + 2 P per group (geometry, the weights'
+// recurrence, its normalisation) and 29 + P per strip (its sine and cosine,
+// its P sums).  FP64 plans keep one strip per group (two strips' FP64 sums
+// and weights would not fit the kernel's registers at two waves per SIMD):
+// about 41 + 2 P per group and 42 + P per strip, FP64 instructions, with the
+// orders whose truncation stays below 1e-15.
 constexpr double kChebLowQ = 128.0;
 
-void cheb_strips(const Prepared &p, StructPlan &sp) {
-    const int *orders = kChebOrders;
-    static double limit[kChebNumOrders] = {0};
+void cheb_strips(const Prepared &p, StructPlan &sp, bool fp64) {
+    static double limit[kChebNumOrders] = {0}, limit64[kCheb64NumOrders] = {0};
     static std::once_flag once;
     std::call_once(once, [] {
         for (int k = 0; k < kChebNumOrders; ++k) limit[k] = cheb_beta_limit(kChebOrders[k]);
+        for (int k = 0; k < kCheb64NumOrders; ++k) limit64[k] = cheb64_beta_limit(kCheb64Orders[k]);
     });
+    const int *orders = fp64 ? kCheb64Orders : kChebOrders;
+    const double *limits = fp64 ? limit64 : limit;
+    const int n_orders = fp64 ? kCheb64NumOrders : kChebNumOrders;
     const int nq = p.nq;
     // the distance bound rounded up to a step of 2^(1/8): structures of about
     // the same extent (jittered frames of one topology, say) get one plan
     const double rplan = std::exp2(std::ceil(8.0 * std::log2(std::max(p.rmax, 1e-6))) / 8.0);
     double best_cost = 0;
-    for (int k = 0; k < kChebNumOrders; ++k) {
-        const double hmax = limit[k] / rplan;
+    for (int k = 0; k < n_orders; ++k) {
+        const double hmax = limits[k] / rplan;
         std::vector<int32_t> first{0};
         std::vector<double> geo, nzs;
         int m0 = 0;
@@ -647,7 +655,7 @@ void cheb_strips(const Prepared &p, StructPlan &sp) {
         const int ns = (int)first.size() - 1;
         std::vector<int32_t> groups{0};
         for (int st = 0; st < ns;) {
-            if (st + 1 < ns) {
+            if (st + 1 < ns && !fp64) {
                 const double hg = std::max(geo[2 * st + 1], geo[2 * st + 3]);
                 const double nzg = std::min(nzs[st], nzs[st + 1]);
                 if (nzg == HUGE_VAL || hg <= kChebLowQ * nzg) {
@@ -660,8 +668,9 @@ void cheb_strips(const Prepared &p, StructPlan &sp) {
             st += 1;
             groups.push_back(st);
         }
-        const double cost = (double)(groups.size() - 1) * (32.0 + 4.0 * orders[k]) +
-                            (double)ns * (22.0 + orders[k]);
+        const double cost =
+            fp64 ? (double)(groups.size() - 1) * (41.0 + 2.0 * orders[k]) + (double)ns * (42.0 + orders[k])
+                 : (double)(groups.size() - 1) * (30.0 + 2.0 * orders[k]) + (double)ns * (29.0 + orders[k]);
         if (k == 0 || cost < best_cost) {
             best_cost = cost;
             sp.P = orders[k];
@@ -672,28 +681,27 @@ void cheb_strips(const Prepared &p, StructPlan &sp) {
     }
 }
 
-// Uniform grids: the longest strip whose accumulators fit, so that one
-// recurrence seed serves more q-points -- FP32 128 (the dual kernel), FP64 64
-// -- unless the grid is short or the structure small.  A small structure
-// alone is latency-bound and its longer strips leave too few items to fill
-// the GPU: FP32 takes 128 from 3e8 pair-q terms (N^2 nq / 2) on, where the
-// two plans measured even (1,200 atoms x 500 q; 64 was 16 % faster at 1,200 x
-// 200, 128 16 % faster at 1,200 x 800).  Non-uniform grids take a direct sine
-// per term and keep 32.
+// From 3e7 pair-q terms (N^2 nq / 2) on, both precisions take the Chebyshev
+// kernels on any grid (kChebQ).  Measured against the best recurrence plan
+// (pair kernel, one box), FP32: 1,000 atoms x 500 q 0.066 against 0.101 ms,
+// 2,000 x 200 0.088 against 0.127, 10,000 x 500 0.634 against 1.706, 30,000
+// x 800 6.8 against 22.9, and on a log grid 10,000 x 1,000 1.30 against 7.79
+// ms; FP64: 1,000 x 500 0.055 against 0.133 ms, 10,000 x 500 0.96 against
+// 4.32 ms.  Only the smallest structures (300 x 200: FP32 0.077 against
+// 0.054 ms, FP64 0.061 against 0.051) are faster with the short recurrence
+// strips.
 //
-// FP32 from 3e7 pair-q terms on takes the Chebyshev kernel on any grid
-// (kChebQ).  Measured against the best recurrence plan (pair kernel, one
-// box): 1,000 atoms x 500 q 0.066 against 0.101 ms, 2,000 x 200 0.088
-// against 0.127, 10,000 x 500 0.634 against 1.706, 30,000 x 800 6.8 against
-// 22.9, and on a log grid 10,000 x 1,000 1.30 against 7.79 ms; only the
-// smallest structures (300 x 200: 0.077 against 0.054 ms) are faster with
-// the short recurrence strips.
+// Below that, a small structure alone is latency-bound: uniform FP32 grids
+// of more than 32 q-points take 64-point strips, everything else 32 (a
+// direct sine per term on non-uniform grids).  The longer recurrence strips
+// (FP32 128, FP64 64) only won past the threshold; they stay for explicit
+// plans.
+constexpr double kChebTerms = 3e7;
 int default_q(int precision, bool uniform, int nq, int n_atoms) {
     const double terms = 0.5 * (double)n_atoms * (double)n_atoms * (double)nq;
-    if (precision == PRIPPS_FP32_COMPENSATED && terms >= 3e7) return kChebQ;
-    if (!uniform || nq <= 32) return 32;
-    if (precision == PRIPPS_FP64) return n_atoms < 2048 ? 32 : 64;
-    return (nq <= 64 || terms < 3e8) ? 64 : 128;
+    if (terms >= kChebTerms) return kChebQ;
+    if (!uniform || nq <= 32 || precision == PRIPPS_FP64) return 32;
+    return 64;
 }
 
 // j atoms per work item: about 16 items per wave slot of the GPU when the
@@ -758,6 +766,7 @@ int log2_nq_of(int nq) {
 // persistent launch.
 struct LaunchGroup {
     bool uniform = true;
+    bool fp64 = false;  // Chebyshev plans: the FP64 kernel (one strip per group)
     int Q = 64, variant = 3, P = 0;
     std::vector<int> members;
     std::vector<Item> items;
@@ -812,7 +821,7 @@ static StructPlan resolve_plan(pripps_engine *e, const pripps_tuning *tune, int 
                  : have ? valid_q(precision, p.uniform, te.q_per_lane) : 0;
     if (!Q) Q = default_q(precision, p.uniform, p.nq, p.n);
     sp.Q = Q;
-    if (Q == kChebQ) cheb_strips(p, sp);
+    if (Q == kChebQ) cheb_strips(p, sp, precision == PRIPPS_FP64);
     const int base = default_span(p, Q, e->n_cu, Q == kChebQ ? sp.n_units(p.nq) : 0);
     if (expl && tune->span > 0) sp.span = std::max(64, tune->span);
     else if (have) sp.span = shifted_span(base, te.span_shift);
@@ -926,10 +935,12 @@ static void plan_cheb_group(LaunchGroup &g, const std::vector<Prepared> &P,
         std::vector<double> f(ng);
         double fs = 0.0;
         for (int u = 0; u < ng; ++u) {
-            f[u] = 32.0 + 4.0 * q.P + q.unit_strips(u) * (22.0 + q.P);
+            f[u] = g.fp64 ? 41.0 + 2.0 * q.P + q.unit_strips(u) * (42.0 + q.P)
+                          : 30.0 + 2.0 * q.P + q.unit_strips(u) * (29.0 + q.P);
             fs += f[u];
         }
-        const double fref = 32.0 + 4.0 * q.P + 2.0 * (22.0 + q.P);
+        const double fref = g.fp64 ? 41.0 + 2.0 * q.P + 2.0 * (42.0 + q.P)
+                                   : 30.0 + 2.0 * q.P + 2.0 * (29.0 + q.P);
         const double wref = (double)W * fs / fref;  // cost in j atoms of a two-strip group
         const int64_t us =
             q.span_set ? std::max<int64_t>(1, (int64_t)std::ceil(wref / q.span))
@@ -990,7 +1001,7 @@ static void plan_cheb_group(LaunchGroup &g, const std::vector<Prepared> &P,
     // Combos (structure, strip group, i kind, j kind), numbered in that
     // order, list their records in queue order (a counting sort), so a
     // structure's lists do not depend on the batch.
-    g.RS = (2 * g.P + 63) / 64 * 64;
+    g.RS = ((g.fp64 ? 1 : 2) * g.P + 63) / 64 * 64;
     const int nm = (int)g.members.size();
     std::vector<int> local(P.size(), -1);
     std::vector<int64_t> cbase(nm + 1, 0);
@@ -1165,13 +1176,14 @@ static void plan_build(pripps_engine *e, const std::vector<Prepared> &P, int pre
                 g->uniform = P[s].uniform;
                 g->variant = B.sp[s].variant;
                 g->P = B.sp[s].P;
+                g->fp64 = B.fp64;
             }
             g->members.push_back(s);
         }
         for (auto &g : B.groups) {
             if (g.Q == kChebQ) {
                 int bpc = 1;
-                if (cheb_occupancy(g.P, &bpc) != hipSuccess) bpc = 1;
+                if ((g.fp64 ? cheb64_occupancy(g.P, &bpc) : cheb_occupancy(g.P, &bpc)) != hipSuccess) bpc = 1;
                 plan_cheb_group(g, P, B.sp, e->n_cu * 4 * std::max(1, bpc));
                 B.n_items += (int64_t)g.units.size();
             } else {
@@ -1354,7 +1366,7 @@ static int make_launches(pripps_engine *e, const BuildPlan &B, char *dbase,
             c.n_st = (int)g.st.size();
         }
         int bpc = 1;
-        if (g.Q == kChebQ) HIPCHK(cheb_occupancy(g.P, &bpc));
+        if (g.Q == kChebQ) HIPCHK(g.fp64 ? cheb64_occupancy(g.P, &bpc) : cheb_occupancy(g.P, &bpc));
         else HIPCHK(tiled_occupancy(B.fp64, g.Q, g.uniform, g.variant, &bpc));
         bpc = std::max(1, bpc);
         const int64_t want = (r.n_items + 3) / 4;

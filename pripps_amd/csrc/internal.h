@@ -25,6 +25,9 @@ constexpr int kChebSets = 4;
 // expansion orders the Chebyshev kernel is built for (ascending)
 constexpr int kChebOrders[] = {12, 16, 20, 22, 24, 26, 28, 30, 32, 36, 40};
 constexpr int kChebNumOrders = sizeof(kChebOrders) / sizeof(kChebOrders[0]);
+// ... and its FP64 form, whose orders keep the truncation below 1e-15
+constexpr int kCheb64Orders[] = {16, 20, 24, 28, 32, 36};
+constexpr int kCheb64NumOrders = sizeof(kCheb64Orders) / sizeof(kCheb64Orders[0]);
 
 // One atom of the sorted SoA-in-AoS stream: centred coordinates and the
 // hydration weight, 16 bytes (FP32) or 32 bytes (FP64) per atom so a wave
@@ -115,6 +118,12 @@ struct ChebLimits {
     float z_floor, k1, b1, k2, b2;
 };
 
+// The same for the FP64 Chebyshev kernel (cheb64_limits): z clamped to
+// z_floor, the start at order P + 2 is clamp(fma(z, k1, b1), 0, 2^-600).
+struct ChebLimits64 {
+    double z_floor, k1, b1;
+};
+
 // Kernel argument blocks.
 struct TiledArgs {
     const DevStruct *structs;
@@ -125,6 +134,7 @@ struct TiledArgs {
     int variant;       // FP32 uniform inner-loop variant (see debye_tiled_kernel)
     const Unit *units; // Chebyshev plans, else nullptr
     ChebLimits cheb;   // Chebyshev plans: set by launch_cheb for the launch's order
+    ChebLimits64 cheb64;
 };
 
 struct FinalizeArgs {
@@ -167,6 +177,12 @@ hipError_t launch_cheb(int P, const TiledArgs &a, int blocks, hipStream_t st);
 hipError_t cheb_occupancy(int P, int *blocks_per_cu);
 // start values of the pair recurrence for order P; false if P is not supported
 bool cheb_limits(int P, ChebLimits *out);
+// FP64 Chebyshev pair kernel, its occupancy, start values and truncation limit
+// (1e-15 of a pair's amplitude)
+hipError_t launch_cheb64(int P, const TiledArgs &a, int blocks, hipStream_t st);
+hipError_t cheb64_occupancy(int P, int *blocks_per_cu);
+bool cheb64_limits(int P, ChebLimits64 *out);
+double cheb64_beta_limit(int P);
 hipError_t launch_cheb_reduce(const ChebReduceArgs &a, hipStream_t st);
 // smallest built order whose truncation is below 1e-8 of a pair's amplitude
 // at beta = H r up to beta_max (0: none)

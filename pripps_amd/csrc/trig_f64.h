@@ -24,32 +24,61 @@ constexpr double kDSinH[12] = {6.283185307179586, -41.34170224039976, 81.6052492
                                -0.012031585942120627, 0.0011309237482517963,
                                -8.823533599243006e-05};
 
+// The quarter-turn coefficients as values.  A kernel may hold them in
+// registers the compiler cannot see through (an empty asm with a "+v"
+// operand): with constants it evaluates each Horner step as a two-address
+// FMA after a 64-bit copy of the coefficient, with register values as one
+// three-address FMA.
+struct SinCosCoeffs {
+    double s[9];
+    double c[8];
+};
+
+__device__ __forceinline__ SinCosCoeffs sincos_coeffs_f64() {
+    SinCosCoeffs k;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) k.s[i] = kDSinQ[i];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) k.c[i] = kDCosQ[i];
+    return k;
+}
+
 // sin(2 pi t) and cos(2 pi t) - 1 for |t| <= 1/8 revolution.
-__device__ __forceinline__ void sincos_poly_f64(double t, double &s, double &cm1) {
+__device__ __forceinline__ void sincos_poly_f64(double t, double &s, double &cm1,
+                                                const SinCosCoeffs &k) {
     const double t2 = t * t;
-    double ps = kDSinQ[8];
+    double ps = k.s[8];
 #pragma unroll
-    for (int i = 7; i >= 0; --i) ps = fma(ps, t2, kDSinQ[i]);
+    for (int i = 7; i >= 0; --i) ps = fma(ps, t2, k.s[i]);
     s = t * ps;
-    double pc = kDCosQ[7];
+    double pc = k.c[7];
 #pragma unroll
-    for (int i = 6; i >= 0; --i) pc = fma(pc, t2, kDCosQ[i]);
+    for (int i = 6; i >= 0; --i) pc = fma(pc, t2, k.c[i]);
     cm1 = t2 * pc;
+}
+
+__device__ __forceinline__ void sincos_poly_f64(double t, double &s, double &cm1) {
+    sincos_poly_f64(t, s, cm1, sincos_coeffs_f64());
 }
 
 // sin(2 pi x) and cos(2 pi x) for x in revolutions: exact reduction to the
 // nearest quarter turn, odd (sine) and even (cosine) in x bit for bit.
-__device__ __forceinline__ void sincos_rev_f64(double x, double &s, double &c) {
+__device__ __forceinline__ void sincos_rev_f64(double x, double &s, double &c,
+                                               const SinCosCoeffs &kc) {
     const double k = rint(4.0 * x);
     const double t = fma(k, -0.25, x);
     double st, cm1;
-    sincos_poly_f64(t, st, cm1);
+    sincos_poly_f64(t, st, cm1, kc);
     const double ct = 1.0 + cm1;
     const int qd = (int)k;
     const double a = (qd & 1) ? ct : st;
     const double b = (qd & 1) ? st : ct;
     s = (qd & 2) ? -a : a;
     c = ((qd + 1) & 2) ? -b : b;
+}
+
+__device__ __forceinline__ void sincos_rev_f64(double x, double &s, double &c) {
+    sincos_rev_f64(x, s, c, sincos_coeffs_f64());
 }
 
 // sin(2 pi x) for x in revolutions: exact reduction to the nearest half turn,

@@ -82,13 +82,14 @@ def test_nonuniform_q(engine, oracle, precision):
 
 
 @pytest.mark.parametrize("precision,n,nq,want_q", [
-    # FP32: the Chebyshev kernel (256) from 3e7 pair-q terms (N^2 nq / 2);
-    # below, 64-point strips when nq > 32, else 32
+    # Both precisions take the Chebyshev kernel (256) from 3e7 pair-q terms
+    # (N^2 nq / 2); below, FP32 64-point strips when nq > 32, else 32, and
+    # FP64 32
     ("fp32", 500, 240, 256), ("fp32", 500, 239, 64), ("fp32", 1000, 61, 256),
     ("fp32", 1000, 59, 64), ("fp32", 700, 20, 32), ("fp32", 2000, 150, 256),
     ("fp32", 2048, 32, 256),
-    # FP64: 64-point strips from 2,048 atoms when nq > 32, else 32
-    ("fp64", 2047, 129, 32), ("fp64", 2048, 129, 64), ("fp64", 2048, 32, 32),
+    ("fp64", 1000, 59, 32), ("fp64", 1000, 61, 256), ("fp64", 500, 240, 256),
+    ("fp64", 500, 239, 32), ("fp64", 2048, 20, 256),
 ])
 def test_default_plan_boundaries(engine, oracle, precision, n, nq, want_q):
     """The default kernel and strip length at their boundaries in N and nq."""
