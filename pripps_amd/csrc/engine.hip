@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <tuple>
 #include <climits>
 #include <chrono>
 #include <cmath>
@@ -64,7 +65,7 @@ static int hip_fail(hipError_t e, const char *what) {
     } while (0)
 
 extern "C" const char *pripps_last_error(void) { return g_last_error.c_str(); }
-extern "C" const char *pripps_version(void) { return "pripps_amd 0.2.0 (gfx950)"; }
+extern "C" const char *pripps_version(void) { return "pripps_amd 0.3.0 (gfx950)"; }
 
 // ---------------------------------------------------------------------------
 // device block pool for model buffers
@@ -851,6 +852,30 @@ static StructPlan resolve_plan(pripps_engine *e, const pripps_tuning *tune, int 
 // depends on the structure alone, so its profiles are bitwise the same alone
 // or in any batch.  The queue runs the units costliest first (a stable sort).
 constexpr int64_t kChebUnitMin = 48;
+#ifndef PRIPPS_RANK_W3
+#define PRIPPS_RANK_W3 1.6, 1.0, 0.5
+#endif
+#ifndef PRIPPS_RANK_W2
+#define PRIPPS_RANK_W2 1.6, 0.5
+#endif
+
+// Unit weights by wave age.  A SIMD issues the oldest ready wave first, so of
+// the waves that share a SIMD -- one from each of the `ranks` blocks resident
+// on a CU -- the oldest runs almost as if alone and the youngest gets what is
+// left: with equal units at 3 blocks per CU they ended at about 117, 176 and
+// 246 us of a 0.27 ms launch (solvent10k, FP32).  The dispatcher places the
+// first n_cu blocks one per CU, then the next n_cu, so block b's age rank on
+// its CU is b / n_cu (97.5 % of the waves, measured), and wave slot g runs
+// unit g first.  A structure that fills the launch is therefore cut into
+// units in the repeating weight pattern of its rank count: the costliest-
+// first order puts the heaviest third on the oldest blocks.
+static double cheb_rank_weight(int ranks, int64_t k) {
+    static const double w3[3] = {PRIPPS_RANK_W3};
+    static const double w2[2] = {PRIPPS_RANK_W2};
+    if (ranks == 3) return w3[k % 3];
+    if (ranks == 2) return w2[k % 2];
+    return 1.0;
+}
 
 // The same cut for two structures: same atoms per class, tiles, strips and
 // order (frames of one topology, say).
@@ -870,11 +895,12 @@ static bool same_cut(const Prepared &a, const StructPlan &qa, const Prepared &b,
 }
 
 static void plan_cheb_group(LaunchGroup &g, const std::vector<Prepared> &P,
-                            const std::vector<StructPlan> &sp, int n_waves) {
+                            const std::vector<StructPlan> &sp, int n_waves, int ranks) {
     struct Pending {
         double cost;
         Unit u;
         int32_t s, strip;
+        int32_t cls;  // wave-age class of a rank-weighted unit (0: heaviest), else 0
     };
     std::vector<Pending> pend;
     g.items.clear();
@@ -946,17 +972,29 @@ static void plan_cheb_group(LaunchGroup &g, const std::vector<Prepared> &P,
             q.span_set ? std::max<int64_t>(1, (int64_t)std::ceil(wref / q.span))
                        : std::max<int64_t>(1, std::min<int64_t>(n_waves, (int64_t)(wref / (double)kChebUnitMin)));
         g.unit_len = std::max(g.unit_len, (int64_t)std::ceil(wref / (double)us));
+        // a structure that fills the launch (one unit per wave slot) is cut
+        // into units weighted by wave age, classes in turn over all its
+        // units, so that each class holds a third (for three ranks) of them
+        const bool by_rank = !q.span_set && us == n_waves && (ranks == 2 || ranks == 3);
+        int64_t kk = 0;  // the structure's unit count so far
         for (int u = 0; u < ng; ++u) {
             const int nk = q.unit_strips(u);
             const int strip = q.unit_first(u);
             int64_t ug = std::max<int64_t>(1, std::llround((double)us * f[u] / fs));
             ug = std::min(ug, W);
+            std::vector<double> cw((size_t)ug);
+            double acc_w = 0.0;
+            for (int64_t k = 0; k < ug; ++k) {
+                acc_w += by_rank ? cheb_rank_weight(ranks, kk + k) : 1.0;
+                cw[(size_t)k] = acc_w;
+            }
             size_t t = 0;
             int32_t j = p.tiles[0].x;
             int64_t rest = cost(j, p.n);  // cost of the current tile from j on
             int64_t done = 0, prev = 0;
             for (int64_t k = 0; k < ug && t < p.tiles.size(); ++k) {
-                const int64_t target = k + 1 == ug ? W : W * (k + 1) / ug;
+                const int64_t target =
+                    k + 1 == ug ? W : std::min<int64_t>(W, (int64_t)((double)W * cw[(size_t)k] / acc_w));
                 const int32_t item0 = (int32_t)g.items.size();
                 while (done < target && t < p.tiles.size()) {
                     const int4 &tl = p.tiles[t];
@@ -980,9 +1018,11 @@ static void plan_cheb_group(LaunchGroup &g, const std::vector<Prepared> &P,
                 }
                 const int32_t item1 = (int32_t)g.items.size();
                 if (item1 > item0)
-                    pend.push_back(Pending{(double)(done - prev) * f[u], Unit{item0, item1, 0, nk}, s, strip});
+                    pend.push_back(Pending{(double)(done - prev) * f[u], Unit{item0, item1, 0, nk}, s, strip,
+                                           by_rank ? (int32_t)((kk + k) % ranks) : 0});
                 prev = done;
             }
+            kk += ug;
         }
         prev_s = s;
         prev_items0 = items0;
@@ -990,17 +1030,43 @@ static void plan_cheb_group(LaunchGroup &g, const std::vector<Prepared> &P,
         prev_pend0 = pend0;
         prev_pend1 = pend.size();
     }
-    // costliest first, ties in generation order
-    std::vector<std::pair<double, int32_t>> order(pend.size());
-    for (size_t x = 0; x < order.size(); ++x) order[x] = {-pend[x].cost, (int32_t)x};
-    std::sort(order.begin(), order.end());
-    g.units.resize(pend.size());
-    for (size_t x = 0; x < order.size(); ++x) g.units[x] = pend[order[x].second].u;
-    // Records: one per class run of each item, unit by unit in queue order
-    // (the order in which the kernel numbers them from the unit's rec0).
-    // Combos (structure, strip group, i kind, j kind), numbered in that
-    // order, list their records in queue order (a counting sort), so a
-    // structure's lists do not depend on the batch.
+    // Queue order: costliest first, ties in generation order; with rank-
+    // weighted units, per class c (heaviest first) its costliest n_waves /
+    // ranks units first -- the units that wave slots [c, c + 1) n_waves /
+    // ranks, the blocks of age rank c, take without the queue -- and then
+    // every other unit costliest first.
+    bool weighted = false;
+    for (const Pending &pd : pend) weighted |= pd.cls != 0;
+    std::vector<int32_t> order;
+    order.reserve(pend.size());
+    {
+        std::vector<std::pair<double, int32_t>> byc(pend.size());
+        for (size_t x = 0; x < pend.size(); ++x) byc[x] = {-pend[x].cost, (int32_t)x};
+        std::sort(byc.begin(), byc.end());
+        if (weighted) {
+            const int64_t per = n_waves / ranks;
+            std::vector<char> first(pend.size(), 0);
+            for (int c = 0; c < ranks; ++c) {
+                int64_t taken = 0;
+                for (const auto &e : byc)
+                    if (taken < per && pend[e.second].cls == c) {
+                        order.push_back(e.second);
+                        first[e.second] = 1;
+                        ++taken;
+                    }
+            }
+            for (const auto &e : byc)
+                if (!first[e.second]) order.push_back(e.second);
+        } else {
+            for (const auto &e : byc) order.push_back(e.second);
+        }
+    }
+    // Records: one per class run of each item, numbered unit by unit in
+    // generation order (a unit's records are consecutive from its rec0, in
+    // the order in which the kernel writes them).  Combos (structure, strip
+    // group, i kind, j kind), numbered in that order, list their records in
+    // record order (a counting sort).  Neither depends on the queue order or
+    // on the batch, so a structure's sums are the same alone or in any batch.
     g.RS = ((g.fp64 ? 1 : 2) * g.P + 63) / 64 * 64;
     const int nm = (int)g.members.size();
     std::vector<int> local(P.size(), -1);
@@ -1020,7 +1086,8 @@ static void plan_cheb_group(LaunchGroup &g, const std::vector<Prepared> &P,
             for (int st = q.unit_first(u); st < q.unit_first(u) + q.unit_strips(u); ++st) group_of[k][st] = u;
     }
     std::vector<int32_t> rec_combo;
-    for (Unit &u : g.units) {
+    for (Pending &pd : pend) {
+        Unit &u = pd.u;
         u.rec0 = (int32_t)rec_combo.size();
         for (int32_t i = u.item0; i < u.item1; ++i) {
             const Item &it = g.items[i];
@@ -1035,6 +1102,8 @@ static void plan_cheb_group(LaunchGroup &g, const std::vector<Prepared> &P,
         }
     }
     g.n_records = (int64_t)rec_combo.size();
+    g.units.resize(pend.size());
+    for (size_t x = 0; x < order.size(); ++x) g.units[x] = pend[order[x]].u;
     const int64_t n_combos = cbase[nm];
     std::vector<int32_t> cptr(n_combos + 1, 0);
     for (int32_t c : rec_combo) cptr[c + 1]++;
@@ -1184,7 +1253,7 @@ static void plan_build(pripps_engine *e, const std::vector<Prepared> &P, int pre
             if (g.Q == kChebQ) {
                 int bpc = 1;
                 if ((g.fp64 ? cheb64_occupancy(g.P, &bpc) : cheb_occupancy(g.P, &bpc)) != hipSuccess) bpc = 1;
-                plan_cheb_group(g, P, B.sp, e->n_cu * 4 * std::max(1, bpc));
+                plan_cheb_group(g, P, B.sp, e->n_cu * 4 * std::max(1, bpc), std::max(1, bpc));
                 B.n_items += (int64_t)g.units.size();
             } else {
                 plan_group(g, P, B.sp);
