@@ -729,24 +729,22 @@ __global__ void __launch_bounds__(kChebQ) cheb_combo_kernel(ChebReduceArgs a) {
     for (int pr = 0; pr < kProfiles; ++pr) o[pr * kChebQ] = v[pr];
 }
 
-// Stage 3: block t adds the combo contributions of its strip in order,
-// divides by q and adds the diagonal (q = 0: closed form).
+// Stage 3: block (t, pr) adds the profile-pr contributions of strip t's
+// combos in order, divides by q and adds the diagonal (q = 0: closed form).
+// One block per profile: each sum is a chain of dependent loads, and six of
+// them in one block took six times as long.
 __global__ void __launch_bounds__(kChebQ) cheb_strip_kernel(ChebReduceArgs a) {
-    const int4 st = a.st[blockIdx.x];
+    const int4 st = a.st[blockIdx.x / kProfiles];
+    const int pr = blockIdx.x % kProfiles;
     const PRIPPS_CONST DevStruct &sd = as_const(a.structs)[st.x];
     const int strip = st.y, ql = threadIdx.x;
     const int m = sd.strip_first[strip] + ql;
     if (m >= sd.strip_first[strip + 1]) return;
-    double acc[kProfiles];
-#pragma unroll
-    for (int pr = 0; pr < kProfiles; ++pr)
-        acc[pr] = sum_strided(a.sb_out + pr * kChebQ + ql, st.z, st.w, kProfiles * kChebQ);
+    const double acc = sum_strided(a.sb_out + pr * kChebQ + ql, st.z, st.w, kProfiles * kChebQ);
     double diag[kProfiles], zero[kProfiles];
     diagonal_and_zero(sd, m, diag, zero);
     const double q = sd.q[m];
-#pragma unroll
-    for (int pr = 0; pr < kProfiles; ++pr)
-        sd.profiles[pr * sd.nq + m] = q > 0.0 ? acc[pr] / q + diag[pr] : zero[pr];
+    sd.profiles[pr * sd.nq + m] = q > 0.0 ? acc / q + diag[pr] : zero[pr];
 }
 
 hipError_t launch_cheb_reduce(const ChebReduceArgs &a, hipStream_t st) {
@@ -761,7 +759,7 @@ hipError_t launch_cheb_reduce(const ChebReduceArgs &a, hipStream_t st) {
         if (e != hipSuccess) return e;
     }
     if (a.n_st > 0) {
-        hipLaunchKernelGGL(cheb_strip_kernel, dim3(a.n_st), dim3(kChebQ), 0, st, a);
+        hipLaunchKernelGGL(cheb_strip_kernel, dim3(a.n_st * kProfiles), dim3(kChebQ), 0, st, a);
         return hipGetLastError();
     }
     return hipSuccess;
