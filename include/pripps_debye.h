@@ -85,8 +85,9 @@ typedef struct pripps_debye_model pripps_debye_model;
 /* Kernel tuning; zero fields mean "choose automatically". */
 typedef struct {
     int32_t q_per_lane;   /* q points per lane strip: 32, 64 or 128 (FP64: 32, 64);
-                             256: the FP32 Chebyshev kernel (strips of up to 256
-                             q-points, any grid) */
+                             256: the Chebyshev kernel (FP32 or FP64, strips of up
+                             to 256 q-points, any grid; the default from 3e7
+                             pair-q terms, N^2 nq / 2) */
     int32_t block;        /* pairs summed in FP32 before the FP64 carry */
     int32_t span;         /* atoms per work item along j; Chebyshev plans: j atoms per
                              work unit (0: one unit per wave slot of the GPU) */

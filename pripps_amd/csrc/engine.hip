@@ -852,12 +852,6 @@ static StructPlan resolve_plan(pripps_engine *e, const pripps_tuning *tune, int 
 // depends on the structure alone, so its profiles are bitwise the same alone
 // or in any batch.  The queue runs the units costliest first (a stable sort).
 constexpr int64_t kChebUnitMin = 48;
-#ifndef PRIPPS_RANK_W3
-#define PRIPPS_RANK_W3 1.6, 1.0, 0.5
-#endif
-#ifndef PRIPPS_RANK_W2
-#define PRIPPS_RANK_W2 1.6, 0.5
-#endif
 
 // Unit weights by wave age.  A SIMD issues the oldest ready wave first, so of
 // the waves that share a SIMD -- one from each of the `ranks` blocks resident
@@ -869,9 +863,12 @@ constexpr int64_t kChebUnitMin = 48;
 // unit g first.  A structure that fills the launch is therefore cut into
 // units in the repeating weight pattern of its rank count: the costliest-
 // first order puts the heaviest third on the oldest blocks.
+// Weights measured best on solvent10k (FP32, P = 26: 0.2523 ms against
+// 0.2655 equal, 0.2701 at 2.2 : 1.1 : 0.35) and its FP64 plan; helix60k (P =
+// 40) gains a further 3 % from 2.2 : 1.1 : 0.35.
 static double cheb_rank_weight(int ranks, int64_t k) {
-    static const double w3[3] = {PRIPPS_RANK_W3};
-    static const double w2[2] = {PRIPPS_RANK_W2};
+    static constexpr double w3[3] = {1.6, 1.0, 0.5};
+    static constexpr double w2[2] = {1.6, 0.5};
     if (ranks == 3) return w3[k % 3];
     if (ranks == 2) return w2[k % 2];
     return 1.0;
