@@ -864,12 +864,14 @@ constexpr int64_t kChebUnitMin = 48;
 // units in the repeating weight pattern of its rank count: the costliest-
 // first order puts the heaviest third on the oldest blocks.
 // Weights measured best on solvent10k (FP32, P = 26: 0.2523 ms against
-// 0.2655 equal, 0.2701 at 2.2 : 1.1 : 0.35) and its FP64 plan; helix60k (P =
-// 40) gains a further 3 % from 2.2 : 1.1 : 0.35.
-static double cheb_rank_weight(int ranks, int64_t k) {
+// 0.2655 equal, 0.2701 at 2.2 : 1.1 : 0.35) and its FP64 plan, and on
+// helix60k (P = 40: 51.2 ms at 2.2 : 1.1 : 0.35, 52.9 at 1.6 : 1.0 : 0.5,
+// 57.0 equal).
+static double cheb_rank_weight(int ranks, int P, int64_t k) {
     static constexpr double w3[3] = {1.6, 1.0, 0.5};
+    static constexpr double w3_long[3] = {2.2, 1.1, 0.35};
     static constexpr double w2[2] = {1.6, 0.5};
-    if (ranks == 3) return w3[k % 3];
+    if (ranks == 3) return P >= 36 ? w3_long[k % 3] : w3[k % 3];
     if (ranks == 2) return w2[k % 2];
     return 1.0;
 }
@@ -982,7 +984,7 @@ static void plan_cheb_group(LaunchGroup &g, const std::vector<Prepared> &P,
             std::vector<double> cw((size_t)ug);
             double acc_w = 0.0;
             for (int64_t k = 0; k < ug; ++k) {
-                acc_w += by_rank ? cheb_rank_weight(ranks, kk + k) : 1.0;
+                acc_w += by_rank ? cheb_rank_weight(ranks, g.P, kk + k) : 1.0;
                 cw[(size_t)k] = acc_w;
             }
             size_t t = 0;
