@@ -705,17 +705,22 @@ __global__ void __launch_bounds__(kChebQ) cheb_combo_kernel(ChebReduceArgs a) {
     if (m < end) {
         const double qc = sd.strip_geo[2 * strip], H = sd.strip_geo[2 * strip + 1];
         const double x = H > 0.0 ? (qm - qc) / H : 0.0;
-        double s[kChebSets];
-        for (int set = 0; set < kChebSets; ++set) {
-            const double *c = coef + set * a.RS + sa.z * a.P;
-            double b1 = 0.0, b2 = 0.0;
-            for (int p = a.P - 1; p >= 1; --p) {
-                const double b0 = __builtin_fma(2.0 * x, b1, c[p] - b2);
-                b2 = b1;
-                b1 = b0;
+        // the four sets' Clenshaw recurrences side by side (independent chains)
+        double s[kChebSets], b1[kChebSets], b2[kChebSets];
+        const double *c = coef + sa.z * a.P;
+#pragma unroll
+        for (int set = 0; set < kChebSets; ++set) b1[set] = b2[set] = 0.0;
+        for (int p = a.P - 1; p >= 1; --p) {
+#pragma unroll
+            for (int set = 0; set < kChebSets; ++set) {
+                const double b0 = __builtin_fma(2.0 * x, b1[set], c[set * a.RS + p] - b2[set]);
+                b2[set] = b1[set];
+                b1[set] = b0;
             }
-            s[set] = __builtin_fma(x, b1, 0.5 * c[0] - b2);  // c_0 holds w_0 = 2 u_0
         }
+#pragma unroll
+        for (int set = 0; set < kChebSets; ++set)  // c_0 holds w_0 = 2 u_0
+            s[set] = __builtin_fma(x, b1[set], 0.5 * c[set * a.RS] - b2[set]);
         // sets: S = sum s, Si = sum w_i s, Sw = sum w_j s, Siw = sum w_i w_j s
         v[0] = 2.0 * ak * al * s[0];
         v[1] = 2.0 * ek * el * s[0];
