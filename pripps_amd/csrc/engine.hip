@@ -65,7 +65,7 @@ static int hip_fail(hipError_t e, const char *what) {
     } while (0)
 
 extern "C" const char *pripps_last_error(void) { return g_last_error.c_str(); }
-extern "C" const char *pripps_version(void) { return "pripps_amd 0.3.0 (gfx950)"; }
+extern "C" const char *pripps_version(void) { return "pripps_amd 0.3.1 (gfx950)"; }
 
 // ---------------------------------------------------------------------------
 // device block pool for model buffers
@@ -864,13 +864,14 @@ constexpr int64_t kChebUnitMin = 48;
 // units in the repeating weight pattern of its rank count: the costliest-
 // first order puts the heaviest third on the oldest blocks.
 // Weights measured best on solvent10k (FP32, P = 26: 0.2523 ms against
-// 0.2655 equal, 0.2701 at 2.2 : 1.1 : 0.35) and its FP64 plan, and on
+// 0.2655 equal, 0.2701 at 2.2 : 1.1 : 0.35; FP64, P = 36: 0.681-0.684 ms at
+// 1.8 : 0.4 against 0.693 at 1.6 : 0.5, 0.706 at 2.0 : 0.35), and on
 // helix60k (P = 40: 51.2 ms at 2.2 : 1.1 : 0.35, 52.9 at 1.6 : 1.0 : 0.5,
 // 57.0 equal).
 static double cheb_rank_weight(int ranks, int P, int64_t k) {
     static constexpr double w3[3] = {1.6, 1.0, 0.5};
     static constexpr double w3_long[3] = {2.2, 1.1, 0.35};
-    static constexpr double w2[2] = {1.6, 0.5};
+    static constexpr double w2[2] = {1.8, 0.4};
     if (ranks == 3) return P >= 36 ? w3_long[k % 3] : w3[k % 3];
     if (ranks == 2) return w2[k % 2];
     return 1.0;
